@@ -1,0 +1,506 @@
+// C ABI of the MI355X EEMD engine (declared in include/eemd_mi355x.h).
+//
+// Host-side responsibilities: validate arguments, lay out the work (signal and
+// task descriptors, queue order longest-first), size and reuse device scratch
+// owned by the context, enqueue fill -> persistent sift -> mean kernels, and
+// time the sift and mean kernels with HIP events on the launch stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/eemd_mi355x.h"
+#include "eemd_common.h"
+
+namespace eemd {
+hipError_t launch_sift(const SiftArgs& A, int fp64, int nt, int grid, hipStream_t st);
+hipError_t sift_occupancy(int fp64, int nt, int* blocks_per_cu);
+hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start,
+                       const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
+                       int max_imf, const double* x, double* avg, double* resid,
+                       int32_t* nmodes, int64_t total_n, hipStream_t st);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
+hipError_t launch_fill_bytes(void* p, int64_t n, hipStream_t st);
+hipError_t launch_noise(uint64_t seed, uint32_t sig, uint32_t m0, int E, int64_t n, double* out,
+                        hipStream_t st);
+}  // namespace eemd
+
+using namespace eemd;
+
+namespace {
+
+// A device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t b = std::max<size_t>(want, 256);
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    bytes = b;
+    return hipSuccess;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+}  // namespace
+
+struct eemd_ctx {
+  int device = 0;
+  int ncu = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  std::string err;
+  // device buffers owned by the context
+  DevBuf scratch, acc, acc_cnt, sigd, taskd, order, sig_start, queue;
+  // staging for the host-pointer entry point
+  DevBuf hx, hnoise, havg, hres, hnm, hnmode, hnsift, hpm, hmi, htr;
+  double last_sift_ms = 0.0, last_mean_ms = 0.0;
+  int last_threads = 0, last_grid = 0, last_ntask = 0;
+};
+
+namespace {
+
+int fail(eemd_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(eemd_ctx* c, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  (void)hipGetLastError();
+  return fail(c, e == hipErrorOutOfMemory ? EEMD_ERR_OOM : EEMD_ERR_DEVICE, m);
+}
+
+#define HIPCHK(c, expr, what)                 \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) return hip_fail(c, _e, what); \
+  } while (0)
+
+struct Plan {
+  std::vector<SignalDesc> sig;
+  std::vector<TaskDesc> task;
+  std::vector<int32_t> order;
+  std::vector<int64_t> start;  // prefix of lengths (mean kernel)
+  int64_t total_n = 0;
+  int64_t nmax = 0;
+  int64_t acc_doubles = 0;
+  int64_t units = 0;
+  int threads = 0;
+  int grid = 0;
+  int64_t npad = 0, cap = 0, rows = 0, cols = 0;
+  int64_t stride = 0;
+};
+
+int auto_threads(int64_t nmax) {
+  if (nmax <= 8192) return 256;
+  if (nmax <= 65536) return 512;
+  return 1024;
+}
+
+int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const double* sigmas,
+             const eemd_params* p) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  if (!p) return fail(c, EEMD_ERR_BAD_ARG, "params is NULL");
+  if (!offsets || nsig < 1) return fail(c, EEMD_ERR_BAD_ARG, "need at least one signal");
+  if (E < 1) return fail(c, EEMD_ERR_BAD_ARG, "ensemble size E must be >= 1");
+  if (!sigmas) return fail(c, EEMD_ERR_BAD_ARG, "sigmas is NULL");
+  if (p->max_imf < 1 || p->max_imf > MAX_IMF_CAP)
+    return fail(c, EEMD_ERR_BAD_ARG, "max_imf must be in [1, 64]");
+  if (p->max_sifts < 0) return fail(c, EEMD_ERR_BAD_ARG, "max_sifts must be >= 0");
+  if (p->end_mode < 0 || p->end_mode > 2) return fail(c, EEMD_ERR_BAD_ARG, "bad end_mode");
+  if (p->end_mode == EEMD_END_LEARNED)
+    return fail(c, EEMD_ERR_BAD_ARG, "learned boundary needs a model (eemd_run_learned)");
+  if (p->halo < 0) return fail(c, EEMD_ERR_BAD_ARG, "halo must be >= 0");
+  if (p->threads != 0 && p->threads != 256 && p->threads != 512 && p->threads != 1024)
+    return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 256, 512 or 1024");
+  if (p->members_per_task < 0) return fail(c, EEMD_ERR_BAD_ARG, "members_per_task must be >= 0");
+  if (offsets[0] != 0) return fail(c, EEMD_ERR_BAD_ARG, "offsets[0] must be 0");
+  for (int s = 0; s < nsig; ++s) {
+    const int64_t n = offsets[s + 1] - offsets[s];
+    if (n < 3) return fail(c, EEMD_ERR_BAD_ARG, "every signal needs at least 3 samples");
+    if (n > (int64_t(1) << 30)) return fail(c, EEMD_ERR_BAD_ARG, "signal longer than 2^30");
+    if (!p->fp64 && n > (int64_t(1) << 24))
+      return fail(c, EEMD_ERR_BAD_ARG, "FP32 mode holds sample positions exactly only up to 2^24");
+  }
+  if ((int64_t)nsig * E > (int64_t(1) << 30)) return fail(c, EEMD_ERR_BAD_ARG, "too many members");
+  return EEMD_OK;
+}
+
+int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const double* sigmas,
+              const double* noise, const eemd_params* p, bool want_mimf, Plan& P) {
+  const int tb = p->fp64 ? 8 : 4;
+  P.nmax = 0;
+  for (int s = 0; s < nsig; ++s) P.nmax = std::max<int64_t>(P.nmax, offsets[s + 1] - offsets[s]);
+  P.threads = p->threads ? p->threads : auto_threads(P.nmax);
+  const int nw = P.threads / WAVE;
+
+  int bpc = 1;
+  hipError_t oe = sift_occupancy(p->fp64, P.threads, &bpc);
+  if (oe != hipSuccess) return hip_fail(c, oe, "occupancy query");
+  if (bpc < 1) bpc = 1;
+  const int64_t max_slots = (int64_t)bpc * c->ncu;
+
+  // members per task: enough tasks to fill every slot twice, but accumulators
+  // stay proportional to the number of tasks
+  int mpt = p->members_per_task;
+  if (mpt <= 0) {
+    const int64_t units = (int64_t)nsig * E;
+    const int64_t target = std::max<int64_t>(2 * max_slots, nsig);
+    mpt = (int)std::max<int64_t>(1, units / target);
+    mpt = std::min(mpt, E);
+  }
+
+  P.sig.resize(nsig);
+  P.start.resize(nsig);
+  P.task.clear();
+  int64_t noise_off = 0, avg_off = 0, mimf_off = 0, acc_off = 0, tot = 0;
+  for (int s = 0; s < nsig; ++s) {
+    const int64_t n = offsets[s + 1] - offsets[s];
+    SignalDesc& d = P.sig[s];
+    d.x_off = offsets[s];
+    d.n = n;
+    d.noise_off = noise ? noise_off : -1;
+    d.avg_off = avg_off;
+    d.mimf_off = want_mimf ? mimf_off : -1;
+    d.sigma = sigmas[s];
+    d.unit0 = s * E;
+    d.task0 = (int32_t)P.task.size();
+    d.nmember = E;
+    for (int m0 = 0; m0 < E; m0 += mpt) {
+      TaskDesc t{};
+      t.signal = s;
+      t.m0 = m0;
+      t.m1 = std::min(E, m0 + mpt);
+      t.acc_off = acc_off;
+      acc_off += (int64_t)p->max_imf * n;
+      P.task.push_back(t);
+    }
+    d.ntask = (int32_t)P.task.size() - d.task0;
+    P.start[s] = tot;
+    tot += n;
+    noise_off += (int64_t)E * n;
+    avg_off += (int64_t)p->max_imf * n;
+    mimf_off += (int64_t)E * p->max_imf * n;
+  }
+  P.total_n = tot;
+  P.acc_doubles = acc_off;
+  P.units = (int64_t)nsig * E;
+  P.order.resize(P.task.size());
+  std::iota(P.order.begin(), P.order.end(), 0);
+  std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) {
+    const auto& ta = P.task[a];
+    const auto& tbk = P.task[b];
+    const int64_t wa = P.sig[ta.signal].n * (ta.m1 - ta.m0);
+    const int64_t wb = P.sig[tbk.signal].n * (tbk.m1 - tbk.m0);
+    return wa > wb;
+  });
+  P.grid = (int)std::min<int64_t>((int64_t)P.task.size(), max_slots);
+
+  // scratch geometry
+  P.npad = ((P.nmax + CHUNK - 1) / CHUNK) * CHUNK + CHUNK;
+  P.cap = seg_cap(seg_len(P.nmax, nw));
+  if (p->block <= 0) {
+    P.rows = P.npad / 2 + 2 * MAX_VIRT + 4;
+    P.cols = 2;
+  } else {
+    P.rows = (int64_t)p->block + 2 * (int64_t)p->halo;
+    P.cols = P.threads;
+  }
+  P.stride = make_layout(tb, P.npad, nw, P.cap, P.rows, P.cols).total;
+  return EEMD_OK;
+}
+
+int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
+             const eemd_params* p, double* avg, double* resid, int32_t* n_modes, int32_t* nmode,
+             int32_t* nsift, int32_t* per_mode, void* mimf, uint64_t* trace, hipStream_t st,
+             bool sync_timing) {
+  const size_t nt = P.task.size();
+  HIPCHK(c, c->scratch.ensure((size_t)P.stride * P.grid), "scratch alloc");
+  HIPCHK(c, c->acc.ensure(sizeof(double) * (size_t)std::max<int64_t>(P.acc_doubles, 1)), "acc alloc");
+  HIPCHK(c, c->acc_cnt.ensure(sizeof(int32_t) * nt * p->max_imf), "acc_cnt alloc");
+  HIPCHK(c, c->sigd.ensure(sizeof(SignalDesc) * P.sig.size()), "desc alloc");
+  HIPCHK(c, c->taskd.ensure(sizeof(TaskDesc) * nt), "desc alloc");
+  HIPCHK(c, c->order.ensure(sizeof(int32_t) * nt), "desc alloc");
+  HIPCHK(c, c->sig_start.ensure(sizeof(int64_t) * P.start.size()), "desc alloc");
+  HIPCHK(c, c->queue.ensure(256), "queue alloc");
+  HIPCHK(c, hipMemcpyAsync(c->sigd.p, P.sig.data(), sizeof(SignalDesc) * P.sig.size(),
+                           hipMemcpyHostToDevice, st), "desc upload");
+  HIPCHK(c, hipMemcpyAsync(c->taskd.p, P.task.data(), sizeof(TaskDesc) * nt,
+                           hipMemcpyHostToDevice, st), "desc upload");
+  HIPCHK(c, hipMemcpyAsync(c->order.p, P.order.data(), sizeof(int32_t) * nt,
+                           hipMemcpyHostToDevice, st), "desc upload");
+  HIPCHK(c, hipMemcpyAsync(c->sig_start.p, P.start.data(), sizeof(int64_t) * P.start.size(),
+                           hipMemcpyHostToDevice, st), "desc upload");
+  HIPCHK(c, launch_fill_i32((int32_t*)c->queue.p, 1, 0, st), "queue reset");
+  HIPCHK(c, launch_fill_f64(avg, (int64_t)p->max_imf * P.total_n, 0.0, st), "mean clear");
+  const int tb = p->fp64 ? 8 : 4;
+  if (mimf) {
+    int64_t cnt = (int64_t)P.units * p->max_imf;  // per-unit blocks differ in n
+    int64_t bytes = 0;
+    for (const auto& s : P.sig) bytes += (int64_t)s.nmember * p->max_imf * s.n * tb;
+    (void)cnt;
+    HIPCHK(c, launch_fill_bytes(mimf, bytes, st), "member imf clear");
+  }
+  if (trace) {
+    const int64_t words = P.units * p->max_imf * std::max(p->max_sifts, 1) * 4;
+    HIPCHK(c, launch_fill_bytes(trace, words * 8, st), "trace clear");
+  }
+
+  SiftArgs A{};
+  A.x = x;
+  A.noise = noise;
+  A.sig = (const SignalDesc*)c->sigd.p;
+  A.task = (const TaskDesc*)c->taskd.p;
+  A.order = (const int32_t*)c->order.p;
+  A.ntask = (int32_t)nt;
+  A.nsig = (int32_t)P.sig.size();
+  A.queue = (int32_t*)c->queue.p;
+  A.seed = p->seed;
+  A.max_imf = p->max_imf;
+  A.max_sifts = p->max_sifts;
+  A.min_extrema = p->min_extrema;
+  A.end_mode = p->end_mode;
+  A.tau = p->tau;
+  A.B = p->block;
+  A.S = p->halo;
+  A.acc = (double*)c->acc.p;
+  A.acc_cnt = (int32_t*)c->acc_cnt.p;
+  A.member_imfs = mimf;
+  A.nmode = nmode;
+  A.nsift = nsift;
+  A.per_mode = per_mode;
+  A.trace = trace;
+  A.scratch = (char*)c->scratch.p;
+  A.scratch_stride = P.stride;
+  A.npad = P.npad;
+  A.cap = P.cap;
+  A.rows = P.rows;
+  A.cols = P.cols;
+  A.mlp = nullptr;
+  A.nslots = P.grid;
+
+  HIPCHK(c, hipEventRecord(c->ev[0], st), "event");
+  HIPCHK(c, launch_sift(A, p->fp64, P.threads, P.grid, st), "sift kernel launch");
+  HIPCHK(c, hipEventRecord(c->ev[1], st), "event");
+  HIPCHK(c, launch_mean((const SignalDesc*)c->sigd.p, (int)P.sig.size(),
+                        (const int64_t*)c->sig_start.p, (const TaskDesc*)c->taskd.p,
+                        (const double*)c->acc.p, (const int32_t*)c->acc_cnt.p, p->max_imf, x, avg,
+                        resid, n_modes, P.total_n, st),
+         "mean kernel launch");
+  HIPCHK(c, hipEventRecord(c->ev[2], st), "event");
+  c->last_threads = P.threads;
+  c->last_grid = P.grid;
+  c->last_ntask = (int)nt;
+  if (sync_timing) {
+    HIPCHK(c, hipStreamSynchronize(st), "kernel execution");
+    float a = 0.f, b = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "event timing");
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "event timing");
+    c->last_sift_ms = a;
+    c->last_mean_ms = b;
+  }
+  return EEMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void eemd_params_default(eemd_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->fp64 = 1;
+  p->end_mode = EEMD_END_MIRROR;
+  p->max_imf = 10;
+  p->max_sifts = 10;
+  p->min_extrema = 4;
+  p->block = 1024;
+  p->halo = 30;
+  p->threads = 0;
+  p->members_per_task = 0;
+  p->flags = 0;
+  p->tau = 0.0;
+  p->seed = 0;
+}
+
+int eemd_ctx_create(int device, eemd_ctx** out) {
+  if (!out) return EEMD_ERR_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    return EEMD_ERR_DEVICE;
+  }
+  if (device < 0 || device >= ndev) return EEMD_ERR_BAD_ARG;
+  eemd_ctx* c = new eemd_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess ||
+      hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipStreamCreate(&c->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return EEMD_ERR_DEVICE;
+  }
+  for (auto& e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      (void)hipGetLastError();
+      delete c;
+      return EEMD_ERR_DEVICE;
+    }
+  }
+  *out = c;
+  return EEMD_OK;
+}
+
+int eemd_ctx_destroy(eemd_ctx* c) {
+  if (!c) return EEMD_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->sigd, &c->taskd, &c->order,
+                    &c->sig_start, &c->queue, &c->hx, &c->hnoise, &c->havg, &c->hres, &c->hnm,
+                    &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr})
+    b->release();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return EEMD_OK;
+}
+
+const char* eemd_last_error(const eemd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int eemd_run_batch_dev(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t nsig,
+                       int32_t E, const double* sigmas, const double* noise,
+                       const eemd_params* p, double* avg, double* resid, int32_t* n_modes,
+                       int32_t* nmode, int32_t* nsift, int32_t* per_mode, void* mimf,
+                       uint64_t* trace, void* stream) {
+  int rc = validate(c, offsets, nsig, E, sigmas, p);
+  if (rc) return rc;
+  if (!x || !avg || !resid || !n_modes || !nmode || !nsift)
+    return fail(c, EEMD_ERR_BAD_ARG, "required device buffer is NULL");
+  if (mimf && !(p->flags & EEMD_FLAG_MEMBER_IMFS))
+    return fail(c, EEMD_ERR_BAD_ARG, "member_imfs given without EEMD_FLAG_MEMBER_IMFS");
+  if (trace && !(p->flags & EEMD_FLAG_TRACE))
+    return fail(c, EEMD_ERR_BAD_ARG, "trace given without EEMD_FLAG_TRACE");
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  Plan P;
+  rc = make_plan(c, offsets, nsig, E, sigmas, noise, p, mimf != nullptr, P);
+  if (rc) return rc;
+  // per_mode is required by the kernel; use a context buffer when absent
+  int32_t* pm = per_mode;
+  if (!pm) {
+    HIPCHK(c, c->hpm.ensure(sizeof(int32_t) * P.units * p->max_imf), "per-mode alloc");
+    pm = (int32_t*)c->hpm.p;
+  }
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  return run_plan(c, P, x, noise, p, avg, resid, n_modes, nmode, nsift, pm, mimf, trace, st,
+                  stream == nullptr);
+}
+
+int eemd_run_batch(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t nsig, int32_t E,
+                   const double* sigmas, const double* noise, const eemd_params* p, double* avg,
+                   double* resid, int32_t* n_modes, int32_t* nmode, int32_t* nsift,
+                   int32_t* per_mode, void* mimf, uint64_t* trace) {
+  int rc = validate(c, offsets, nsig, E, sigmas, p);
+  if (rc) return rc;
+  if (!x || !avg || !resid || !n_modes || !nmode || !nsift)
+    return fail(c, EEMD_ERR_BAD_ARG, "required output buffer is NULL");
+  if (mimf && !(p->flags & EEMD_FLAG_MEMBER_IMFS))
+    return fail(c, EEMD_ERR_BAD_ARG, "member_imfs given without EEMD_FLAG_MEMBER_IMFS");
+  if (trace && !(p->flags & EEMD_FLAG_TRACE))
+    return fail(c, EEMD_ERR_BAD_ARG, "trace given without EEMD_FLAG_TRACE");
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  Plan P;
+  rc = make_plan(c, offsets, nsig, E, sigmas, noise, p, mimf != nullptr, P);
+  if (rc) return rc;
+  const int tb = p->fp64 ? 8 : 4;
+  const int64_t ntot = P.total_n;
+  const int64_t navg = (int64_t)p->max_imf * ntot;
+  int64_t nmi = 0;
+  for (const auto& s : P.sig) nmi += (int64_t)E * p->max_imf * s.n;
+  const int64_t ntr = P.units * p->max_imf * std::max(p->max_sifts, 1) * 4;
+  hipStream_t st = c->stream;
+  HIPCHK(c, c->hx.ensure(sizeof(double) * ntot), "input alloc");
+  if (noise) HIPCHK(c, c->hnoise.ensure(sizeof(double) * ntot * E), "noise alloc");
+  HIPCHK(c, c->havg.ensure(sizeof(double) * navg), "output alloc");
+  HIPCHK(c, c->hres.ensure(sizeof(double) * ntot), "output alloc");
+  HIPCHK(c, c->hnm.ensure(sizeof(int32_t) * nsig), "output alloc");
+  HIPCHK(c, c->hnmode.ensure(sizeof(int32_t) * P.units), "output alloc");
+  HIPCHK(c, c->hnsift.ensure(sizeof(int32_t) * P.units), "output alloc");
+  HIPCHK(c, c->hpm.ensure(sizeof(int32_t) * P.units * p->max_imf), "output alloc");
+  if (mimf) HIPCHK(c, c->hmi.ensure((size_t)tb * nmi), "member imf alloc");
+  if (trace) HIPCHK(c, c->htr.ensure(sizeof(uint64_t) * ntr), "trace alloc");
+  HIPCHK(c, hipMemcpyAsync(c->hx.p, x, sizeof(double) * ntot, hipMemcpyHostToDevice, st), "H2D x");
+  if (noise)
+    HIPCHK(c, hipMemcpyAsync(c->hnoise.p, noise, sizeof(double) * ntot * E, hipMemcpyHostToDevice, st),
+           "H2D noise");
+  rc = run_plan(c, P, (const double*)c->hx.p, noise ? (const double*)c->hnoise.p : nullptr, p,
+                (double*)c->havg.p, (double*)c->hres.p, (int32_t*)c->hnm.p, (int32_t*)c->hnmode.p,
+                (int32_t*)c->hnsift.p, (int32_t*)c->hpm.p, mimf ? c->hmi.p : nullptr,
+                trace ? (uint64_t*)c->htr.p : nullptr, st, true);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(avg, c->havg.p, sizeof(double) * navg, hipMemcpyDeviceToHost, st), "D2H");
+  HIPCHK(c, hipMemcpyAsync(resid, c->hres.p, sizeof(double) * ntot, hipMemcpyDeviceToHost, st), "D2H");
+  HIPCHK(c, hipMemcpyAsync(n_modes, c->hnm.p, sizeof(int32_t) * nsig, hipMemcpyDeviceToHost, st), "D2H");
+  HIPCHK(c, hipMemcpyAsync(nmode, c->hnmode.p, sizeof(int32_t) * P.units, hipMemcpyDeviceToHost, st), "D2H");
+  HIPCHK(c, hipMemcpyAsync(nsift, c->hnsift.p, sizeof(int32_t) * P.units, hipMemcpyDeviceToHost, st), "D2H");
+  if (per_mode)
+    HIPCHK(c, hipMemcpyAsync(per_mode, c->hpm.p, sizeof(int32_t) * P.units * p->max_imf,
+                             hipMemcpyDeviceToHost, st), "D2H");
+  if (mimf)
+    HIPCHK(c, hipMemcpyAsync(mimf, c->hmi.p, (size_t)tb * nmi, hipMemcpyDeviceToHost, st), "D2H");
+  if (trace)
+    HIPCHK(c, hipMemcpyAsync(trace, c->htr.p, sizeof(uint64_t) * ntr, hipMemcpyDeviceToHost, st), "D2H");
+  HIPCHK(c, hipStreamSynchronize(st), "D2H sync");
+  return EEMD_OK;
+}
+
+int eemd_noise(eemd_ctx* c, uint64_t seed, int32_t signal, int32_t member0, int32_t E, int64_t n,
+               double* out) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  if (!out || E < 1 || n < 1 || signal < 0 || member0 < 0)
+    return fail(c, EEMD_ERR_BAD_ARG, "bad noise request");
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  DevBuf tmp;
+  HIPCHK(c, tmp.ensure(sizeof(double) * (size_t)n * E), "noise alloc");
+  hipError_t e = launch_noise(seed, (uint32_t)signal, (uint32_t)member0, E, n, (double*)tmp.p, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, tmp.p, sizeof(double) * (size_t)n * E, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  tmp.release();
+  if (e != hipSuccess) return hip_fail(c, e, "noise");
+  return EEMD_OK;
+}
+
+int eemd_last_timing(const eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* threads,
+                     int32_t* grid, int32_t* ntasks) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  if (sift_ms) *sift_ms = c->last_sift_ms;
+  if (mean_ms) *mean_ms = c->last_mean_ms;
+  if (threads) *threads = c->last_threads;
+  if (grid) *grid = c->last_grid;
+  if (ntasks) *ntasks = c->last_ntask;
+  return EEMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+/*
+ * eemd_mi355x.h -- C ABI of the MI355X EEMD engine (libeemd_mi355x.so).
+ *
+ * Plain C: pointers, sizes, integer status codes; no exceptions or C++/torch
+ * types cross this boundary.  Every entry point returns EEMD_OK (0) or an
+ * EEMD_ERR_* code; eemd_last_error() gives the message of the context's last
+ * failure.
+ *
+ * What each entry point replaces in the reference (paths relative to the
+ * reference root):
+ *
+ *   eemd_run_batch      gpu_eemd_full(x[E,N], E, tau, max_sifts, max_imf, ...)
+ *                       -> (imfs[E,maxm,N], nsift[E], nmode[E])
+ *                       (emd_gpu_v5_dsbt.py:354-384, emd_gpu.py:301-334),
+ *                       together with the host-side level-wise averaging of
+ *                       emd_par._gpu_decompose (emd_par.py:225-250) and
+ *                       emd_ref.eemd (emd_ref.py:199-217).  One call handles
+ *                       many signals of different lengths (the reference
+ *                       launches once per signal).
+ *   eemd_run_batch_dev  the same on device-resident buffers (no H2D / D2H).
+ *   eemd_noise          the noise matrix of emd_par.py:233 / emd_ref.py:204,
+ *                       here the counter-based normals the kernels generate.
+ *   eemd_ctx_create /   the kernel factory + buffer allocation of
+ *   eemd_ctx_destroy    emd_gpu_v5_dsbt.py:137-168, 343-347, 367-376: device
+ *                       scratch is owned by the context and reused.
+ *
+ * Error conventions (emd_gpu.py:307-315, emd_par.py:281-286): invalid shapes
+ * or parameters -> EEMD_ERR_BAD_ARG (the Python layer raises ValueError);
+ * device failures -> EEMD_ERR_DEVICE; allocation failures -> EEMD_ERR_OOM
+ * (RuntimeError).
+ *
+ * Threading: one context per host thread.  Calls are synchronous on the
+ * context's stream unless the _dev variant is given a stream, in which case
+ * work is enqueued on that stream and the call returns after enqueueing.
+ */
+#ifndef EEMD_MI355X_H
+#define EEMD_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EEMD_OK 0
+#define EEMD_ERR_BAD_ARG 1
+#define EEMD_ERR_DEVICE 2
+#define EEMD_ERR_OOM 3
+
+#define EEMD_END_NATURAL 0 /* emd_ref.py:77-103 end='natural' */
+#define EEMD_END_MIRROR 1  /* emd_ref.py:92-95 end='mirror' */
+#define EEMD_END_LEARNED 2 /* learned boundary extension, mirror fallback */
+
+#define EEMD_FLAG_MEMBER_IMFS 1 /* also return every member's IMFs */
+#define EEMD_FLAG_TRACE 2       /* per-sift extrema counts + index digests */
+
+typedef struct eemd_ctx eemd_ctx;
+
+typedef struct eemd_params {
+  int32_t fp64;             /* 1: FP64 arithmetic, 0: FP32 (spline solve + ensemble sum in FP64) */
+  int32_t end_mode;         /* EEMD_END_* */
+  int32_t max_imf;          /* 1..64 IMFs per member (emd_ref.py:125) */
+  int32_t max_sifts;        /* sifts per IMF cap (emd_ref.py:130) */
+  int32_t min_extrema;      /* stop when the residual has fewer extrema (emd_ref.py:123) */
+  int32_t block;            /* decay-seamed block-Thomas block B; <= 0: exact Thomas */
+  int32_t halo;             /* seam halo S in knots */
+  int32_t threads;          /* workgroup size 256 / 512 / 1024; 0: automatic */
+  int32_t members_per_task; /* ensemble members per queue entry; 0: automatic */
+  int32_t flags;            /* EEMD_FLAG_* */
+  double tau;               /* SD stopping tolerance (emd_ref.py:140-146); <= 0: fixed sifts */
+  uint64_t seed;            /* Philox key when no explicit noise is given */
+} eemd_params;
+
+/* Defaults: FP64, mirror, max_imf 10, max_sifts 10, min_extrema 4, B 1024, S 30,
+ * automatic workgroup / task sizes, tau 0, seed 0. */
+void eemd_params_default(eemd_params* p);
+
+int eemd_ctx_create(int device, eemd_ctx** out);
+int eemd_ctx_destroy(eemd_ctx* ctx);
+const char* eemd_last_error(const eemd_ctx* ctx);
+
+/*
+ * EEMD of nsig signals (concatenated in x; signal s is x[offsets[s] ..
+ * offsets[s+1]), length >= 3) with E members each.  Member e of signal s
+ * decomposes x_s + sigma_s * z_{s,e}, where z is the explicit standard-normal
+ * matrix `noise` (per signal an [E][n_s] block, blocks concatenated in signal
+ * order) or, when noise is NULL, the Philox normals of eemd_noise().
+ *
+ * Outputs (host memory, caller-owned):
+ *   imf_mean   [sum_s max_imf * n_s]  per signal a [max_imf][n_s] block; levels
+ *              >= n_modes[s] are zero
+ *   residual   [sum_s n_s]            x - sum of the mean IMFs (emd_ref.py:217)
+ *   n_modes    [nsig]                 levels produced by at least one member
+ *   nmode      [nsig*E]               IMFs of each member
+ *   nsift      [nsig*E]               total sifts of each member
+ *   per_mode_sifts [nsig*E*max_imf]   optional (may be NULL)
+ *   member_imfs    optional (may be NULL; requires EEMD_FLAG_MEMBER_IMFS):
+ *              per signal an [E][max_imf][n_s] block of double (fp64) or
+ *              float (fp32); levels >= nmode are zero
+ *   trace      optional (may be NULL; requires EEMD_FLAG_TRACE):
+ *              [nsig*E][max_imf][max_sifts][4] uint64 = {maxima count, minima
+ *              count, digest of maxima indices, digest of minima indices} of
+ *              the knots each executed sift used (digest: sum of splitmix64)
+ */
+int eemd_run_batch(eemd_ctx* ctx, const double* x, const int64_t* offsets, int32_t nsig,
+                   int32_t E, const double* sigmas, const double* noise, const eemd_params* prm,
+                   double* imf_mean, double* residual, int32_t* n_modes, int32_t* nmode,
+                   int32_t* nsift, int32_t* per_mode_sifts, void* member_imfs, uint64_t* trace);
+
+/* Same, with x, noise, imf_mean, residual, n_modes, nmode, nsift,
+ * per_mode_sifts, member_imfs and trace in device memory (offsets and sigmas
+ * stay on the host).  stream: hipStream_t, or NULL for the context's stream
+ * (then the call waits for completion). */
+int eemd_run_batch_dev(eemd_ctx* ctx, const double* x, const int64_t* offsets, int32_t nsig,
+                       int32_t E, const double* sigmas, const double* noise,
+                       const eemd_params* prm, double* imf_mean, double* residual,
+                       int32_t* n_modes, int32_t* nmode, int32_t* nsift, int32_t* per_mode_sifts,
+                       void* member_imfs, uint64_t* trace, void* stream);
+
+/* The standard normals member e of signal `signal` receives when no explicit
+ * noise is given: out[e][i] for members member0 .. member0+E-1 (host memory). */
+int eemd_noise(eemd_ctx* ctx, uint64_t seed, int32_t signal, int32_t member0, int32_t E,
+               int64_t n, double* out);
+
+/* Device time (ms, HIP events on the launch stream) of the last call's
+ * persistent sift kernel and mean kernel, and its launch configuration. */
+int eemd_last_timing(const eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* threads,
+                     int32_t* grid, int32_t* ntasks);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EEMD_MI355X_H */

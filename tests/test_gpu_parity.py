@@ -1,0 +1,213 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+FP64 mode uses the oracle's arithmetic, so extrema sets, IMF and sift counts
+must match exactly and IMFs agree to rounding (tolerances written below).
+Golden vectors from the reference's emd_ref.py pin the same cases end to end.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from eemd4cuda_amd import _lib
+    return _lib.engine()
+
+
+def params(fp64=1, end="mirror", max_imf=8, max_sifts=10, tau=0.0, B=0, S=30, threads=0, mpt=1,
+           seed=0):
+    from eemd4cuda_amd import _lib
+    p = _lib.default_params()
+    p.fp64 = fp64
+    p.end_mode = _lib.END_MODES[end]
+    p.max_imf = max_imf
+    p.max_sifts = max_sifts
+    p.tau = tau
+    p.block = B
+    p.halo = S
+    p.threads = threads
+    p.members_per_task = mpt
+    p.seed = seed
+    return p
+
+
+def run1(eng, x, z, sigma, **kw):
+    N = len(x)
+    E = z.shape[0] if z is not None else kw.pop("E")
+    prm = params(**kw)
+    out = eng.run_batch(x, np.array([0, N]), E, np.array([sigma]), prm, noise=z,
+                        member_imfs=True, trace=True)
+    mi = prm.max_imf
+    out["avg2"] = out["avg"].reshape(mi, N)[:out["n_modes"][0]]
+    out["mimf"] = out["member_imfs"].reshape(E, mi, N)
+    return out
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_fp64_three_tone_matches_oracle_and_reference(eng, golden):
+    g = golden("golden_three_tone.npz")
+    x, z, sigma = g["x"], g["z"], float(g["sigma"])
+    print("case: three-tone N=4096 E=64 fp64 exact Thomas", flush=True)
+    out = run1(eng, x, z, sigma, B=0)
+    o = orc.eemd(x, z, sigma, tau=0.0, max_sifts=10, max_imf=8, end="mirror",
+                 member_imfs=True, trace=True)
+    assert np.array_equal(out["nmode"][0], o["nmode"])
+    assert np.array_equal(out["nsift"][0], o["nsift"])
+    assert np.array_equal(out["per_mode"][0], o["per_mode"])
+    assert np.array_equal(out["trace"][0], o["trace"])        # extrema sets of every sift
+    d = np.max(np.abs(out["mimf"] - o["member_imfs"]))
+    print("member IMF max|d| vs oracle:", d)
+    assert d <= 1e-13
+    # against the reference's own output (golden)
+    assert np.array_equal(out["nmode"][0], g["nmode"])
+    assert np.array_equal(out["trace"][0], g["trace"])
+    for m in range(len(g["avg"])):
+        assert rel_l2(out["avg2"][m], g["avg"][m]) <= 1e-10
+    assert np.max(np.abs(out["resid"] - g["resid"])) <= 1e-10
+
+
+@pytest.mark.parametrize("B,S,threads", [(64, 30, 256), (100, 14, 1024), (1024, 30, 512)])
+def test_fp64_dsbt_matches_oracle(eng, B, S, threads):
+    rng = np.random.default_rng(7)
+    N = 20000
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 1500) + 0.4 * np.sin(2 * np.pi * t / 97) + 0.1 * rng.standard_normal(N)
+    z = rng.standard_normal((3, N))
+    sigma = 0.2 * np.std(x)
+    print("case: N=20000 E=3 fp64 DSBT B=%d S=%d threads=%d" % (B, S, threads), flush=True)
+    out = run1(eng, x, z, sigma, B=B, S=S, threads=threads, max_imf=6)
+    o = orc.eemd(x, z, sigma, tau=0.0, max_sifts=10, max_imf=6, end="mirror", B=B, S=S,
+                 member_imfs=True, trace=True)
+    assert np.array_equal(out["nmode"][0], o["nmode"])
+    assert np.array_equal(out["trace"][0], o["trace"])
+    assert np.max(np.abs(out["mimf"] - o["member_imfs"])) <= 1e-13
+    assert np.max(np.abs(out["avg2"] - o["avg"])) <= 1e-13
+
+
+def test_fp64_natural_end_matches_reference(eng, golden):
+    g = golden("golden_chirp_natural.npz")
+    print("case: chirp N=2048 E=8 natural", flush=True)
+    out = run1(eng, g["x"], g["z"], float(g["sigma"]), end="natural", max_imf=6, B=0)
+    assert np.array_equal(out["nmode"][0], g["nmode"])
+    assert np.array_equal(out["per_mode"][0], g["per_mode"])
+    assert np.array_equal(out["trace"][0], g["trace"])
+    assert np.max(np.abs(out["mimf"] - g["member_imfs"])) <= 1e-11
+    assert np.max(np.abs(out["avg2"] - g["avg"])) <= 1e-11
+
+
+@pytest.mark.parametrize("end,tau", [("natural", 0.25), ("mirror", 0.2)])
+def test_fp64_sd_stopping_matches_reference(eng, golden, end, tau):
+    g = golden("golden_chirp_emd.npz")
+    k = "%s_tau" % end
+    x = g["x"]
+    print("case: chirp E=1 tau=%g %s" % (tau, end), flush=True)
+    out = run1(eng, x, np.zeros((1, len(x))), 0.0, end=end, tau=tau, max_sifts=50,
+               max_imf=64, B=0)
+    nm = out["nmode"][0, 0]
+    assert nm == len(g[k + "_per_mode"])
+    assert np.array_equal(out["per_mode"][0, 0, :nm], g[k + "_per_mode"])
+    assert np.array_equal(out["trace"][0, 0, :nm, :50], g[k + "_trace"])
+    assert np.max(np.abs(out["mimf"][0, :nm] - g[k + "_imfs"])) <= 1e-10
+
+
+def test_degenerate_inputs_match_reference(eng, golden):
+    g = golden("golden_degenerate.npz")
+    names = sorted(k[:-2] for k in g.files if k.endswith("_x"))
+    for k in names:
+        x = g[k + "_x"]
+        end = "mirror" if k.endswith("mirror") else "natural"
+        print("case: degenerate", k, flush=True)
+        out = run1(eng, x, np.zeros((1, len(x))), 0.0, end=end, max_imf=6, B=0)
+        nm = out["nmode"][0, 0]
+        assert nm == g[k + "_imfs"].shape[0], k
+        assert np.array_equal(out["per_mode"][0, 0, :nm], g[k + "_per_mode"]), k
+        assert np.allclose(out["mimf"][0, :nm], g[k + "_imfs"], rtol=0, atol=1e-12), k
+
+
+def test_philox_noise_matches_oracle(eng):
+    print("case: philox normals", flush=True)
+    z = eng.noise(2024, 5, 3, 4, 1001)
+    for e in range(4):
+        ref = orc.normals(2024, 5, 3 + e, 1001)
+        assert np.max(np.abs(z[e] - ref)) <= 1e-12
+
+
+def test_in_kernel_noise_equals_materialised_noise(eng):
+    x = np.sin(np.arange(6000) * 0.01) + 0.3 * np.sin(np.arange(6000) * 0.37)
+    sigma = 0.2 * np.std(x)
+    print("case: in-kernel philox vs materialised", flush=True)
+    a = run1(eng, x, None, sigma, E=6, B=0, seed=99)
+    z = eng.noise(99, 0, 0, 6, len(x))
+    b = run1(eng, x, z, sigma, B=0)
+    assert np.array_equal(a["nmode"], b["nmode"])
+    assert np.array_equal(a["mimf"], b["mimf"])
+
+
+def test_fp32_error_vs_fp64_oracle(eng, golden):
+    g = golden("golden_three_tone.npz")
+    x, z, sigma = g["x"], g["z"], float(g["sigma"])
+    print("case: three-tone fp32", flush=True)
+    out = run1(eng, x, z, sigma, fp64=0, B=1024, S=14)
+    errs = [rel_l2(out["avg2"][m], g["avg"][m]) for m in range(min(len(g["avg"]), len(out["avg2"])))]
+    print("fp32 per-IMF rel L2 vs reference:", ["%.2e" % e for e in errs])
+    assert max(errs[:4]) <= 1e-3
+
+
+def test_determinism(eng):
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(9000).cumsum() * 0.05 + np.sin(np.arange(9000) * 0.2)
+    print("case: determinism", flush=True)
+    a = run1(eng, x, None, 0.1, E=12, B=256, S=30, seed=5, mpt=3)
+    b = run1(eng, x, None, 0.1, E=12, B=256, S=30, seed=5, mpt=3)
+    assert np.array_equal(a["avg"], b["avg"]) and np.array_equal(a["resid"], b["resid"])
+    c = run1(eng, x, None, 0.1, E=12, B=256, S=30, seed=6, mpt=3)
+    assert not np.array_equal(a["avg"], c["avg"])
+
+
+def test_batched_uneven_equals_per_signal(eng):
+    from eemd4cuda_amd import signals
+    xs = signals.uneven_batch(5, lo_log2=9, hi_log2=13, seed=9)
+    lens = [len(v) for v in xs]
+    offs = np.concatenate(([0], np.cumsum(lens)))
+    sig = np.array([0.2 * np.std(v) for v in xs])
+    E = 5
+    print("case: batched uneven", lens, flush=True)
+    prm = params(B=128, S=30, max_imf=8)
+    bat = eng.run_batch(np.concatenate(xs), offs, E, sig, prm, member_imfs=False)
+    for s, v in enumerate(xs):
+        one = eng.run_batch(v, np.array([0, len(v)]), E, sig[s:s + 1], params(B=128, S=30, max_imf=8))
+        # Philox counters are keyed by the signal index: re-key the single run
+        z = eng.noise(0, s, 0, E, len(v))
+        one = eng.run_batch(v, np.array([0, len(v)]), E, sig[s:s + 1], params(B=128, S=30, max_imf=8),
+                            noise=z)
+        a = bat["avg"][8 * offs[s]: 8 * offs[s + 1]]
+        assert np.array_equal(a, one["avg"]), s
+        assert np.array_equal(bat["nmode"][s], one["nmode"][0]), s
+        assert np.array_equal(bat["resid"][offs[s]:offs[s + 1]], one["resid"]), s
+
+
+def test_workgroup_sizes_agree(eng):
+    rng = np.random.default_rng(2)
+    x = np.sin(np.arange(30000) * 0.003) + 0.2 * rng.standard_normal(30000)
+    outs = []
+    for nt in (256, 512, 1024):
+        print("case: threads", nt, flush=True)
+        outs.append(run1(eng, x, None, 0.1, E=3, B=256, S=30, threads=nt))
+    for o in outs[1:]:
+        assert np.array_equal(o["mimf"], outs[0]["mimf"])
+
+
+def test_members_per_task_only_reassociates(eng):
+    x = np.sin(np.arange(5000) * 0.02) + 0.5 * np.sin(np.arange(5000) * 0.3)
+    print("case: members per task", flush=True)
+    a = run1(eng, x, None, 0.1, E=16, B=0, mpt=1)
+    b = run1(eng, x, None, 0.1, E=16, B=0, mpt=5)
+    assert np.array_equal(a["mimf"], b["mimf"])
+    assert np.max(np.abs(a["avg"] - b["avg"])) <= 1e-13
