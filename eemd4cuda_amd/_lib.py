@@ -17,6 +17,7 @@ EEMD_OK, EEMD_ERR_BAD_ARG, EEMD_ERR_DEVICE, EEMD_ERR_OOM = 0, 1, 2, 3
 END_MODES = {"natural": 0, "mirror": 1, "learned": 2}
 FLAG_MEMBER_IMFS = 1
 FLAG_TRACE = 2
+FLAG_PHASES = 4
 
 class Params(C.Structure):
     _fields_ = [("fp64", C.c_int32), ("end_mode", C.c_int32), ("max_imf", C.c_int32),
@@ -56,8 +57,9 @@ def load(path: Path | None = None):
     L.eemd_noise.argtypes = [vp, u64, i32, i32, i32, i64, vp]
     L.eemd_last_timing.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32),
                                    C.POINTER(i32), C.POINTER(i32)]
+    L.eemd_last_phases.argtypes = [vp, C.POINTER(dbl)]
     for name in ("eemd_ctx_create", "eemd_ctx_destroy", "eemd_run_batch", "eemd_run_batch_dev",
-                 "eemd_noise", "eemd_last_timing"):
+                 "eemd_noise", "eemd_last_timing", "eemd_last_phases"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -164,7 +166,7 @@ class Engine:
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         sigmas = np.ascontiguousarray(sigmas, dtype=np.float64)
         nsig = len(offsets) - 1
-        prm.flags = 0
+        prm.flags = prm.flags & FLAG_PHASES
         vp = C.c_void_p
         with _lock:
             rc = self.lib.eemd_run_batch_dev(
@@ -182,6 +184,12 @@ class Engine:
                                  int(member0), int(E), int(n), _ptr(out))
         self._check(rc, "eemd_noise")
         return out
+
+    def last_phases(self) -> list:
+        out = (C.c_double * 4)()
+        rc = self.lib.eemd_last_phases(self.h, out)
+        self._check(rc, "eemd_last_phases")
+        return list(out)
 
     def last_timing(self) -> dict:
         a, b = C.c_double(), C.c_double()

@@ -71,11 +71,13 @@ struct eemd_ctx {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   std::string err;
   // device buffers owned by the context
-  DevBuf scratch, acc, acc_cnt, sigd, taskd, order, sig_start, queue;
+  DevBuf scratch, acc, acc_cnt, sigd, taskd, order, sig_start, queue, phase;
   // staging for the host-pointer entry point
   DevBuf hx, hnoise, havg, hres, hnm, hnmode, hnsift, hpm, hmi, htr;
   double last_sift_ms = 0.0, last_mean_ms = 0.0;
   int last_threads = 0, last_grid = 0, last_ntask = 0;
+  bool timing_pending = false;
+  int last_phase_grid = 0;
 };
 
 namespace {
@@ -113,9 +115,10 @@ struct Plan {
 };
 
 int auto_threads(int64_t nmax) {
+  // 1024-thread workgroups cap a lane at 128 VGPRs, below what the fused
+  // sift pass needs; 512 keeps every value in registers
   if (nmax <= 8192) return 256;
-  if (nmax <= 65536) return 512;
-  return 1024;
+  return 512;
 }
 
 int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const double* sigmas,
@@ -299,6 +302,12 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.cols = P.cols;
   A.mlp = nullptr;
   A.nslots = P.grid;
+  A.phase = nullptr;
+  if (p->flags & EEMD_FLAG_PHASES) {
+    HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
+    A.phase = (uint64_t*)c->phase.p;
+  }
+  c->last_phase_grid = (p->flags & EEMD_FLAG_PHASES) ? P.grid : 0;
 
   HIPCHK(c, hipEventRecord(c->ev[0], st), "event");
   HIPCHK(c, launch_sift(A, p->fp64, P.threads, P.grid, st), "sift kernel launch");
@@ -312,6 +321,7 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   c->last_threads = P.threads;
   c->last_grid = P.grid;
   c->last_ntask = (int)nt;
+  c->timing_pending = !sync_timing;
   if (sync_timing) {
     HIPCHK(c, hipStreamSynchronize(st), "kernel execution");
     float a = 0.f, b = 0.f;
@@ -378,7 +388,7 @@ int eemd_ctx_destroy(eemd_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->sigd, &c->taskd, &c->order,
-                    &c->sig_start, &c->queue, &c->hx, &c->hnoise, &c->havg, &c->hres, &c->hnm,
+                    &c->sig_start, &c->queue, &c->phase, &c->hx, &c->hnoise, &c->havg, &c->hres, &c->hnm,
                     &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr})
     b->release();
   for (auto& e : c->ev)
@@ -492,14 +502,39 @@ int eemd_noise(eemd_ctx* c, uint64_t seed, int32_t signal, int32_t member0, int3
   return EEMD_OK;
 }
 
-int eemd_last_timing(const eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* threads,
+int eemd_last_timing(eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* threads,
                      int32_t* grid, int32_t* ntasks) {
   if (!c) return EEMD_ERR_BAD_ARG;
+  if (c->timing_pending) {
+    // the last call was enqueued on a caller stream: wait for its events
+    HIPCHK(c, hipEventSynchronize(c->ev[2]), "event wait");
+    float a = 0.f, b = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "event timing");
+    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "event timing");
+    c->last_sift_ms = a;
+    c->last_mean_ms = b;
+    c->timing_pending = false;
+  }
   if (sift_ms) *sift_ms = c->last_sift_ms;
   if (mean_ms) *mean_ms = c->last_mean_ms;
   if (threads) *threads = c->last_threads;
   if (grid) *grid = c->last_grid;
   if (ntasks) *ntasks = c->last_ntask;
+  return EEMD_OK;
+}
+
+int eemd_last_phases(eemd_ctx* c, double* ms4) {
+  if (!c || !ms4) return EEMD_ERR_BAD_ARG;
+  for (int k = 0; k < 4; ++k) ms4[k] = 0.0;
+  if (c->last_phase_grid <= 0) return fail(c, EEMD_ERR_BAD_ARG, "last call ran without EEMD_FLAG_PHASES");
+  std::vector<uint64_t> h((size_t)c->last_phase_grid * 4);
+  HIPCHK(c, hipStreamSynchronize(c->stream), "phase sync");
+  HIPCHK(c, hipMemcpy(h.data(), c->phase.p, h.size() * sizeof(uint64_t), hipMemcpyDeviceToHost),
+         "phase copy");
+  for (int g = 0; g < c->last_phase_grid; ++g)
+    for (int k = 0; k < 4; ++k) ms4[k] += (double)h[(size_t)g * 4 + k];
+  // wall_clock64 ticks at 100 MHz; report the mean over workgroups in ms
+  for (int k = 0; k < 4; ++k) ms4[k] = ms4[k] / c->last_phase_grid / 1.0e5;
   return EEMD_OK;
 }
 

@@ -35,27 +35,28 @@
 
 namespace eemd {
 
+// A knot value (real or virtual) and the slot of its record.  Virtual knots
+// (end extension) live in the VSLOTS records after the segment regions.
 template <typename T>
 struct Knot {
   T t, y, m;
-  int code;  // >= 0: real knot slot (relative to the envelope region); < 0: virtual
+  int code;
 };
 
 template <typename T, int NW>
 struct Shared {
-  int32_t cnt[2][2][NW];
-  int32_t beg[2][2][NW];
-  int32_t pre[2][2][NW + 1];
+  int32_t cnt[2][2][NW];     // knots per (set, envelope, segment)
+  int32_t beg[2][2][NW];     // first used slot of each segment region
+  int32_t pre[2][2][NW + 1]; // exclusive prefix of cnt over segments
   unsigned long long hsh[2][2];
-  int32_t p[2], nl[2], nr[2], P[2];
-  T cval[2];
-  T vpos[2][2 * MAX_VIRT], vy[2][2 * MAX_VIRT], vm[2][2 * MAX_VIRT];
+  int32_t p[2], nl[2], nr[2], P[2];  // real, left / right virtual, total knots
+  T cval[2];                         // constant envelope when p < 2
+  T vt[2][2 * MAX_VIRT];             // positions of the virtual knots (interval search)
   double red[2][NW];
   int32_t acc_cnt[MAX_IMF_CAP];
   int32_t pm[MAX_IMF_CAP];
   int32_t task;
   double sd;
-  T stage[NW * RPL * WAVE];  // per-wave envelope staging rows
 };
 
 // ------------------------------------------------------------------------
@@ -178,151 +179,226 @@ struct Mem {
   int64_t n, sl, cap;
   T* h;
   T* r;
-  uint8_t* bits;   // [2 sets][2 env][npad/8]
-  int64_t bstride; // npad/8
-  int32_t* kt;     // [2][2][NW*cap]
-  T* ky;
-  T* km;           // [2 env][NW*cap]
-  int64_t rstride; // NW*cap
-  double* wc;
-  double* wx;
-  int32_t* ws;
+  uint8_t* bits;    // [2 sets][2 env][npad/8]
+  int64_t bstride;  // npad/8
+  T* kr;            // knot records [2][2][NW*cap + VSLOTS], KREC_BYTES each
+  int64_t rstride;  // records per (set, env)
+  int64_t vbase;    // slot of virtual knot 0 (= NW*cap)
+  double2* wcx;     // window forward sweep {c', x'} [rows][cols]
+  int32_t* ws;      // slot code of each window row
   int64_t cols;
 };
 
+template <typename T>
+constexpr int rec_len() { return KREC_BYTES / (int)sizeof(T); }
+
+template <typename T>
+__device__ __forceinline__ T* rec(const Mem<T>& M, int set, int env, int64_t slot) {
+  return M.kr + ((int64_t)(set * 2 + env) * M.rstride + slot) * rec_len<T>();
+}
+
+// Slot of real knot g (0-based in time order) of (set, env); w is a segment
+// hint updated in place.
 template <typename T, int NW>
-__device__ __forceinline__ Knot<T> fetch_knot(const Shared<T, NW>& S, const Mem<T>& M, int set,
-                                              int env, int e, int& whint) {
-  Knot<T> k;
-  const int nl = S.nl[env];
-  const int p = S.p[env];
-  if (e < nl) {
-    k.t = S.vpos[env][e];
-    k.y = S.vy[env][e];
-    k.m = S.vm[env][e];
-    k.code = -1 - e;
-    return k;
-  }
-  const int g = e - nl;
-  if (g >= p) {
-    const int j = g - p;
-    k.t = S.vpos[env][MAX_VIRT + j];
-    k.y = S.vy[env][MAX_VIRT + j];
-    k.m = S.vm[env][MAX_VIRT + j];
-    k.code = -1 - MAX_VIRT - j;
-    return k;
-  }
+__device__ __forceinline__ int64_t real_slot(const Shared<T, NW>& S, const Mem<T>& M, int set,
+                                             int env, int g, int& w) {
   const int32_t* pre = S.pre[set][env];
-  int w = whint;
 #pragma unroll 1
   while (w > 0 && g < pre[w]) --w;
 #pragma unroll 1
   while (w < NW - 1 && g >= pre[w + 1]) ++w;
-  whint = w;
-  const int64_t slot = (int64_t)w * M.cap + S.beg[set][env][w] + (g - pre[w]);
-  const int64_t base = (int64_t)(set * 2 + env) * M.rstride;
-  k.t = (T)M.kt[base + slot];
-  k.y = M.ky[base + slot];
-  k.m = M.km[(int64_t)env * M.rstride + slot];
+  return (int64_t)w * M.cap + S.beg[set][env][w] + (g - pre[w]);
+}
+
+// Slot of the knot with extended index e (virtual left knots first, then
+// the real knots in time order, then virtual right knots).
+template <typename T, int NW>
+__device__ __forceinline__ int64_t ext_slot(const Shared<T, NW>& S, const Mem<T>& M, int set,
+                                            int env, int e, int& w) {
+  const int nl = S.nl[env];
+  const int g = e - nl;
+  if (g < 0) return M.vbase + e;
+  const int p = S.p[env];
+  if (g >= p) return M.vbase + MAX_VIRT + (g - p);
+  return real_slot(S, M, set, env, g, w);
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ Knot<T> knot_at(const Shared<T, NW>& S, const Mem<T>& M, int set,
+                                           int env, int e, int& w) {
+  const int64_t slot = ext_slot(S, M, set, env, e, w);
+  const T* r = rec(M, set, env, slot);
+  Knot<T> k;
+  if constexpr (std::is_same<T, float>::value) {
+    const float4 v = *reinterpret_cast<const float4*>(r);
+    k.t = v.x;
+    k.y = v.y;
+    k.m = v.z;
+  } else {
+    const double2 v = *reinterpret_cast<const double2*>(r);
+    k.t = v.x;
+    k.y = v.y;
+    k.m = r[2];
+  }
   k.code = (int)slot;
   return k;
 }
 
-// Interval (extended-knot index of its left knot) of sample i given
-// cl = number of real knots at positions < i: clip(count_ext(<i) - 1, 0, P-2),
-// i.e. searchsorted-left minus one (emd_ref.py:70).
+// Interval (extended index of its left knot) of sample i, given cl = number
+// of real knots at positions < i: clip(count_ext(<i) - 1, 0, P-2), i.e.
+// searchsorted-left minus one (emd_ref.py:70).
 template <typename T, int NW>
-__device__ __forceinline__ int ext_interval(const Shared<T, NW>& S, int env, int cl, int i) {
-  const int nl = S.nl[env], nr = S.nr[env], p = S.p[env];
+__device__ __forceinline__ int ext_interval(const Shared<T, NW>& S, int env, int cl, int i, int nl,
+                                            int nr, int p, int P) {
   int ce = cl + nl;
   if (cl == 0 && nl > 0) {
     ce = 0;
-    for (int j = 0; j < nl; ++j) ce += (S.vpos[env][j] < (T)i) ? 1 : 0;
+    for (int j = 0; j < nl; ++j) ce += (S.vt[env][j] < (T)i) ? 1 : 0;
   }
   if (cl == p && nr > 0) {
-    for (int j = 0; j < nr; ++j) ce += (S.vpos[env][MAX_VIRT + j] < (T)i) ? 1 : 0;
+    for (int j = 0; j < nr; ++j) ce += (S.vt[env][MAX_VIRT + j] < (T)i) ? 1 : 0;
   }
   int e = ce - 1;
-  const int hi = S.P[env] - 2;
   e = e < 0 ? 0 : e;
-  e = e > hi ? hi : e;
+  e = e > P - 2 ? P - 2 : e;
   return e;
 }
 
-// Cubic spline value on [k0, k1] at t (emd_ref.py:71-75).
-__device__ __forceinline__ double spline_val(const Knot<double>& k0, const Knot<double>& k1,
-                                             double t) {
-  const double hh = k1.t - k0.t;
-  const double a = (k1.t - t) / hh;
-  const double b = (t - k0.t) / hh;
+// Cubic spline value on [k0, k1] at t (emd_ref.py:71-75), FP64 order of
+// operations of the reference.
+__device__ __forceinline__ double spline_val(double t0, double y0, double m0, double t1, double y1,
+                                             double m1, double t) {
+  const double hh = t1 - t0;
+  const double a = (t1 - t) / hh;
+  const double b = (t - t0) / hh;
   const double ca = cube_once(a), cb = cube_once(b);
-  return (a * k0.y + b * k1.y) + ((((ca - a) * k0.m) + ((cb - b) * k1.m)) * (hh * hh)) / 6.0;
+  return (a * y0 + b * y1) + ((((ca - a) * m0) + ((cb - b) * m1)) * (hh * hh)) / 6.0;
 }
 
-template <typename T>
-struct Cursor {
-  Knot<T> k0, k1;
-  int e;
-  int wh;
-  T ih, h26;  // float path: 1/h and h*h/6 of the current interval
+// Candidate selection by a per-lane index with plain selects.  The
+// candidates are separate scalars (never an array: the compiler would turn a
+// select chain over an array into indexed scratch-memory loads).
+template <int NC, typename V>
+__device__ __forceinline__ V sel(int j, V a0, V a1, V a2, V a3, V a4, V a5) {
+  V r = a0;
+  if constexpr (NC > 1) r = (j == 1) ? a1 : r;
+  if constexpr (NC > 2) r = (j == 2) ? a2 : r;
+  if constexpr (NC > 3) r = (j == 3) ? a3 : r;
+  if constexpr (NC > 4) r = (j == 4) ? a4 : r;
+  if constexpr (NC > 5) r = (j == 5) ? a5 : r;
+  return r;
+}
+
+// FP64: values of one envelope at samples i0..i0+7; the lane's knots
+// e0..e0+NC-1 are loaded once and selected per sample.
+template <int NC, int NW>
+__device__ __forceinline__ void env_knots(const Shared<double, NW>& S, const Mem<double>& M,
+                                          int cur, int ev, int w, const int (&e)[RPL], int i0,
+                                          double (&out)[RPL]) {
+  const int P = S.P[ev];
+  int wh = w;
+  auto ld = [&](int c) {
+    int ec = e[0] + c;
+    ec = ec > P - 1 ? P - 1 : ec;
+    return knot_at(S, M, cur, ev, ec, wh);
+  };
+  const Knot<double> k0 = ld(0), k1 = ld(1), k2 = ld(NC > 2 ? 2 : 1);
+  const Knot<double> k3 = ld(NC > 3 ? 3 : 1), k4 = ld(NC > 4 ? 4 : 1), k5 = ld(NC > 5 ? 5 : 1);
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int j = e[k] - e[0];
+    const int j1 = j + 1;
+    const double ta = sel<NC>(j, k0.t, k1.t, k2.t, k3.t, k4.t, k5.t);
+    const double ya = sel<NC>(j, k0.y, k1.y, k2.y, k3.y, k4.y, k5.y);
+    const double ma = sel<NC>(j, k0.m, k1.m, k2.m, k3.m, k4.m, k5.m);
+    const double tb = sel<NC>(j1, k0.t, k1.t, k2.t, k3.t, k4.t, k5.t);
+    const double yb = sel<NC>(j1, k0.y, k1.y, k2.y, k3.y, k4.y, k5.y);
+    const double mb = sel<NC>(j1, k0.m, k1.m, k2.m, k3.m, k4.m, k5.m);
+    out[k] = spline_val(ta, ya, ma, tb, yb, mb, (double)(i0 + k));
+  }
+}
+
+// FP32: values of one envelope at samples i0..i0+7.  The lane loads its
+// knots e0..e0+NC, turns each interval into Horner form (the expansion of
+// emd_ref.py:71-75 in tau = t - t0: c1 = (y1-y0)/h - h(2 M0 + M1)/6,
+// c2 = M0/2, c3 = (M1-M0)/(6h)) and selects the interval per sample.
+struct Ivl4 {
+  float t, y, c1, c2, c3;
 };
 
-template <typename T>
-__device__ __forceinline__ void cursor_prep(Cursor<T>& c) {
-  if constexpr (std::is_same<T, float>::value) {
-    const float hh = c.k1.t - c.k0.t;
-    c.ih = 1.0f / hh;
-    c.h26 = hh * hh * (1.0f / 6.0f);
+__device__ __forceinline__ Ivl4 make_ivl(const Knot<float>& a, const Knot<float>& b) {
+  const float hh = b.t - a.t;
+  const float ih = 1.0f / hh;
+  Ivl4 v;
+  v.t = a.t;
+  v.y = a.y;
+  v.c1 = __builtin_fmaf(-hh * (1.0f / 6.0f), __builtin_fmaf(2.0f, a.m, b.m), (b.y - a.y) * ih);
+  v.c2 = 0.5f * a.m;
+  v.c3 = (b.m - a.m) * ih * (1.0f / 6.0f);
+  return v;
+}
+
+template <int NC, int NW>
+__device__ __forceinline__ void env_ivls(const Shared<float, NW>& S, const Mem<float>& M, int cur,
+                                         int ev, int w, const int (&e)[RPL], int i0,
+                                         float (&out)[RPL]) {
+  const int P = S.P[ev];
+  int wh = w;
+  auto ld = [&](int c) {
+    int ec = e[0] + c;
+    ec = ec > P - 1 ? P - 1 : ec;
+    return knot_at(S, M, cur, ev, ec, wh);
+  };
+  const Knot<float> k0 = ld(0), k1 = ld(1);
+  const Knot<float> k2 = ld(NC > 1 ? 2 : 1), k3 = ld(NC > 2 ? 3 : 1);
+  const Knot<float> k4 = ld(NC > 3 ? 4 : 1), k5 = ld(NC > 4 ? 5 : 1);
+  const Ivl4 v0 = make_ivl(k0, k1);
+  const Ivl4 v1 = NC > 1 ? make_ivl(k1, k2) : v0;
+  const Ivl4 v2 = NC > 2 ? make_ivl(k2, k3) : v0;
+  const Ivl4 v3 = NC > 3 ? make_ivl(k3, k4) : v0;
+  const Ivl4 v4 = NC > 4 ? make_ivl(k4, k5) : v0;
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int j = e[k] - e[0];
+    const float t = sel<NC>(j, v0.t, v1.t, v2.t, v3.t, v4.t, v4.t);
+    const float y = sel<NC>(j, v0.y, v1.y, v2.y, v3.y, v4.y, v4.y);
+    const float c1 = sel<NC>(j, v0.c1, v1.c1, v2.c1, v3.c1, v4.c1, v4.c1);
+    const float c2 = sel<NC>(j, v0.c2, v1.c2, v2.c2, v3.c2, v4.c2, v4.c2);
+    const float c3 = sel<NC>(j, v0.c3, v1.c3, v2.c3, v3.c3, v4.c3, v4.c3);
+    const float tau = (float)(i0 + k) - t;
+    out[k] = __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, c3, c2), c1), y);
   }
 }
 
-template <typename T>
-__device__ __forceinline__ T cursor_eval(const Cursor<T>& c, int i) {
-  if constexpr (std::is_same<T, double>::value) {
-    return spline_val(c.k0, c.k1, (double)i);
-  } else {
-    const float t = (float)i;
-    const float a = (c.k1.t - t) * c.ih;
-    const float b = (t - c.k0.t) * c.ih;
-    const float ca = __builtin_fmaf(a, a, -1.0f) * a;
-    const float cb = __builtin_fmaf(b, b, -1.0f) * b;
-    const float lin = __builtin_fmaf(a, c.k0.y, b * c.k1.y);
-    return __builtin_fmaf(__builtin_fmaf(ca, c.k0.m, cb * c.k1.m), c.h26, lin);
-  }
-}
-
-// One envelope's values at samples i0..i0+7, written (ev 0) or added (ev 1)
-// into the wave's LDS staging rows stage[k*64 + lane].  base = number of
-// real knots of set `cur` with bit index < i0; byte = the lane's 8 bits of
-// the current extrema bitmap.  The sample loop is rolled: the interval
-// cursor advances one knot at a time (emd_ref.py:70 searchsorted-left).
+// Values of envelope `ev` (set cur) at the lane's samples i0..i0+7.
+// base = number of real knots with bit index < i0; byte = the lane's bits.
 template <typename T, int NW>
-__device__ __forceinline__ void envelope_chunk(const Shared<T, NW>& S, const Mem<T>& M, int cur,
-                                               int ev, int w, int base, int byte, int i0,
-                                               T* stage) {
-  const int ln = lane_id();
-  if (S.p[ev] < 2) {
-    const T c = S.cval[ev];
-#pragma unroll 1
-    for (int k = 0; k < RPL; ++k) stage[k * WAVE + ln] = ev == 0 ? c : stage[k * WAVE + ln] + c;
+__device__ __forceinline__ void envelope8(const Shared<T, NW>& S, const Mem<T>& M, int cur, int ev,
+                                          int w, int base, int byte, int i0, T (&out)[RPL]) {
+  const int p = S.p[ev];
+  if (p < 2) {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) out[k] = S.cval[ev];
     return;
   }
-  Cursor<T> cs;
-  cs.wh = w;
-  cs.e = -2;
-#pragma unroll 1
-  for (int k = 0; k < RPL; ++k) {
-    const int cl = base + __popc(byte & ((2u << k) - 1u));
-    const int e = ext_interval(S, ev, cl, i0 + k);
-    if (e != cs.e) {
-      if (e != cs.e + 1) cs.k1 = fetch_knot(S, M, cur, ev, e, cs.wh);
-      cs.k0 = cs.k1;
-      cs.k1 = fetch_knot(S, M, cur, ev, e + 1, cs.wh);
-      cs.e = e;
-      cursor_prep(cs);
-    }
-    const T ek = cursor_eval(cs, i0 + k);
-    stage[k * WAVE + ln] = ev == 0 ? ek : stage[k * WAVE + ln] + ek;
+  const int nl = S.nl[ev], nr = S.nr[ev], P = S.P[ev];
+  int e[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k)
+    e[k] = ext_interval(S, ev, base + __popc(byte & ((2u << k) - 1u)), i0 + k, nl, nr, p, P);
+  const int span = e[RPL - 1] - e[0];
+  // wave-uniform number of candidates: strict extrema of one kind are >= 2
+  // samples apart, so 8 samples meet at most 5 intervals
+  const bool need3 = __ballot(span >= 2) != 0ull;
+  const bool need5 = __ballot(span >= 3) != 0ull;
+  if constexpr (std::is_same<T, double>::value) {
+    if (need5) env_knots<6>(S, M, cur, ev, w, e, i0, out);
+    else if (need3) env_knots<4>(S, M, cur, ev, w, e, i0, out);
+    else env_knots<3>(S, M, cur, ev, w, e, i0, out);
+  } else {
+    if (need5) env_ivls<5>(S, M, cur, ev, w, e, i0, out);
+    else if (need3) env_ivls<3>(S, M, cur, ev, w, e, i0, out);
+    else env_ivls<2>(S, M, cur, ev, w, e, i0, out);
   }
 }
 
@@ -345,10 +421,18 @@ struct PassIO {
   int trace;            // accumulate knot digests
 };
 
+template <typename T>
+__device__ __forceinline__ void put_knot(T* r, int q, T val) {
+  if constexpr (std::is_same<T, float>::value)
+    *reinterpret_cast<float2*>(r) = make_float2((float)q, val);
+  else
+    *reinterpret_cast<double2*>(r) = make_double2((double)q, val);
+}
+
 // One wave streams its segment: produce new values, write them, detect the
 // extrema of the new values and compact them into set `nxt`.
 template <typename T, int NW, int MODE>
-__device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, int cur, int nxt) {
+__device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, int cur, int nxt) {
   const int ln = lane_id();
   const int w = (int)(threadIdx.x / WAVE);
   const int64_t n = M.n;
@@ -364,15 +448,12 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
 
   T* const dst = (MODE == PASS_SIFT) ? M.h : M.r;
   const T* const src = io.src_is_r ? M.r : M.h;
-  int32_t* const ktn[2] = {M.kt + (int64_t)(nxt * 2 + 0) * M.rstride + (int64_t)w * M.cap,
-                           M.kt + (int64_t)(nxt * 2 + 1) * M.rstride + (int64_t)w * M.cap};
-  T* const kyn[2] = {M.ky + (int64_t)(nxt * 2 + 0) * M.rstride + (int64_t)w * M.cap,
-                     M.ky + (int64_t)(nxt * 2 + 1) * M.rstride + (int64_t)w * M.cap};
-  uint8_t* const bn[2] = {M.bits + (int64_t)(nxt * 2 + 0) * M.bstride,
-                          M.bits + (int64_t)(nxt * 2 + 1) * M.bstride};
-  const uint8_t* const bc[2] = {M.bits + (int64_t)(cur * 2 + 0) * M.bstride,
-                                M.bits + (int64_t)(cur * 2 + 1) * M.bstride};
+  uint8_t* const bn0 = M.bits + (int64_t)(nxt * 2 + 0) * M.bstride;
+  uint8_t* const bn1 = M.bits + (int64_t)(nxt * 2 + 1) * M.bstride;
+  const uint8_t* const bc0 = M.bits + (int64_t)(cur * 2 + 0) * M.bstride;
+  const uint8_t* const bc1 = M.bits + (int64_t)(cur * 2 + 1) * M.bstride;
   const int kcap = (int)(M.cap - SEG_FRONT);
+  const int64_t seg_slot = (int64_t)w * M.cap + SEG_FRONT;
 
   for (int64_t cb = seg_lo; cb < seg_hi; cb += CHUNK) {
     const int64_t i0 = cb + (int64_t)ln * RPL;
@@ -395,11 +476,11 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
       }
       store8(dst + i0, v);
     } else if constexpr (MODE == PASS_FINAL) {
-      T hv[RPL], rv[RPL];
+      T hv[RPL];
       load8(src + i0, hv);
-      load8(M.r + i0, rv);
+      load8(M.r + i0, v);
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) v[k] = rv[k] - hv[k];
+      for (int k = 0; k < RPL; ++k) v[k] = v[k] - hv[k];
       store8(M.r + i0, v);
       if (io.imf) {
         T* o = reinterpret_cast<T*>(io.imf);
@@ -412,27 +493,27 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
           if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)hv[k];
       }
     } else {
-      // PASS_SIFT / PASS_LAST: mean envelope at the lane's samples first (only
-      // the running sum is live during the evaluation), then h and r.
-      T* const stage = S.stage + w * (RPL * WAVE);
-#pragma unroll
-      for (int ev = 0; ev < 2; ++ev) {
-        const int byte = bc[ev][cb / 8 + ln];
-        int tot;
-        const int ex = wave_excl_small(__popc(byte), tot);
-        const int base = S.pre[cur][ev][w] + crun[ev] + ex;
-        crun[ev] += tot;
-        envelope_chunk<T, NW>(S, M, cur, ev, w, base, byte, (int)i0, stage);
+      // PASS_SIFT / PASS_LAST: mean envelope, then h (and r)
+      T msum[RPL], e1[RPL];
+      {
+        const int b0 = bc0[cb / 8 + ln];
+        const int b1 = bc1[cb / 8 + ln];
+        int t0, t1;
+        const int x0 = wave_excl_small(__popc(b0), t0);
+        const int x1 = wave_excl_small(__popc(b1), t1);
+        const int base0 = S.pre[cur][0][w] + crun[0] + x0;
+        const int base1 = S.pre[cur][1][w] + crun[1] + x1;
+        crun[0] += t0;
+        crun[1] += t1;
+        envelope8(S, M, cur, 0, w, base0, b0, (int)i0, msum);
+        envelope8(S, M, cur, 1, w, base1, b1, (int)i0, e1);
       }
-      T msum[RPL];
-#pragma unroll
-      for (int k = 0; k < RPL; ++k) msum[k] = stage[k * WAVE + ln];
       T hv[RPL];
       load8(src + i0, hv);
       if constexpr (MODE == PASS_SIFT) {
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
-          v[k] = hv[k] - (T)0.5 * msum[k];
+          v[k] = hv[k] - (T)0.5 * (msum[k] + e1[k]);
           if (io.want_sd && i0 + k < n) {
             const T dd = hv[k] - v[k];
             num += (double)(dd * dd);
@@ -442,10 +523,10 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
         store8(M.h + i0, v);
       } else {
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) hv[k] = hv[k] - (T)0.5 * msum[k];  // the IMF
+        for (int k = 0; k < RPL; ++k) hv[k] = hv[k] - (T)0.5 * (msum[k] + e1[k]);  // the IMF
         load8(M.r + i0, v);
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) v[k] = v[k] - hv[k];                // new residual
+        for (int k = 0; k < RPL; ++k) v[k] = v[k] - hv[k];                          // new residual
         store8(M.r + i0, v);
         if (io.imf) {
           T* o = reinterpret_cast<T*>(io.imf);
@@ -481,25 +562,24 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
       if (ok && v0 > vm1 && v0 > vp1) bmax |= 1 << k;
       if (ok && v0 < vm1 && v0 < vp1) bmin |= 1 << k;
     }
-    bn[0][cb / 8 + ln] = (uint8_t)bmax;
-    bn[1][cb / 8 + ln] = (uint8_t)bmin;
+    bn0[cb / 8 + ln] = (uint8_t)bmax;
+    bn1[cb / 8 + ln] = (uint8_t)bmin;
 #pragma unroll
     for (int ev = 0; ev < 2; ++ev) {
       const int bits = ev == 0 ? bmax : bmin;
       int tot;
       const int ex = wave_excl_small(__popc(bits), tot);
       int o = run[ev] + ex;
+      if (bits) {
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) {
-        if ((bits >> k) & 1) {
-          const int q = (int)(i0 + k - 1);
-          const T val = (k == 0) ? p1 : v[k - 1];
-          if (o < kcap) {
-            ktn[ev][SEG_FRONT + o] = q;
-            kyn[ev][SEG_FRONT + o] = val;
+        for (int k = 0; k < RPL; ++k) {
+          if ((bits >> k) & 1) {
+            const int q = (int)(i0 + k - 1);
+            const T val = (k == 0) ? p1 : v[k - 1];
+            if (o < kcap) put_knot(rec(M, nxt, ev, seg_slot + o), q, val);
+            if (io.trace) hs[ev] += mix64((unsigned long long)q);
+            ++o;
           }
-          if (io.trace) hs[ev] += mix64((unsigned long long)q);
-          ++o;
         }
       }
       run[ev] += tot;
@@ -532,7 +612,7 @@ __device__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, i
 // segment; one thread per segment decides them after the pass and inserts
 // them in front of the segment's knot list.
 template <typename T, int NW>
-__device__ __noinline__ void fixup(Shared<T, NW>& S, const Mem<T>& M, const T* vals, int set, int trace) {
+__device__ __forceinline__ void fixup(Shared<T, NW>& S, const Mem<T>& M, const T* vals, int set, int trace) {
   const int w = (int)threadIdx.x;
   if (w < 1 || w >= NW) return;
   const int64_t seg_lo = (int64_t)w * M.sl;
@@ -545,9 +625,7 @@ __device__ __noinline__ void fixup(Shared<T, NW>& S, const Mem<T>& M, const T* v
       const bool hit = ev == 0 ? (b > a && b > c) : (b < a && b < c);
       if (!hit) continue;
       const int bg = S.beg[set][ev][w] - 1;
-      const int64_t base = (int64_t)(set * 2 + ev) * M.rstride + (int64_t)w * M.cap + bg;
-      M.kt[base] = (int32_t)q;
-      M.ky[base] = b;
+      put_knot(rec(M, set, ev, (int64_t)w * M.cap + bg), (int)q, b);
       S.beg[set][ev][w] = bg;
       S.cnt[set][ev][w] += 1;
       uint8_t* bp = M.bits + (int64_t)(set * 2 + ev) * M.bstride + seg_lo / 8;
@@ -558,7 +636,7 @@ __device__ __noinline__ void fixup(Shared<T, NW>& S, const Mem<T>& M, const T* v
 }
 
 template <typename T, int NW>
-__device__ __noinline__ void prefix(Shared<T, NW>& S, int set) {
+__device__ __forceinline__ void prefix(Shared<T, NW>& S, int set) {
   for (int ev = 0; ev < 2; ++ev) {
     int a = 0;
     S.pre[set][ev][0] = 0;
@@ -572,7 +650,7 @@ __device__ __noinline__ void prefix(Shared<T, NW>& S, int set) {
 // Thread 0: envelope bookkeeping of set `cur` (knot counts, end extension,
 // natural boundary M = 0 at the outermost knots).
 template <typename T, int NW>
-__device__ __noinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, int cur, int end_mode) {
+__device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, int cur, int end_mode) {
   for (int ev = 0; ev < 2; ++ev) {
     const int p = S.pre[cur][ev][NW];
     S.p[ev] = p;
@@ -582,46 +660,111 @@ __device__ __noinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, int c
     S.cval[ev] = (T)0;
     int wh = 0;
     if (p == 1) {
-      const Knot<T> k = fetch_knot(S, M, cur, ev, 0, wh);
+      const Knot<T> k = knot_at(S, M, cur, ev, 0, wh);
       S.cval[ev] = k.y;
       continue;
     }
     if (p < 2) continue;
+    const Knot<T> a = knot_at(S, M, cur, ev, 0, wh);
+    const Knot<T> b = knot_at(S, M, cur, ev, 1, wh);
+    int wh2 = NW - 1;
+    const Knot<T> c = knot_at(S, M, cur, ev, p - 2, wh2);
+    const Knot<T> d = knot_at(S, M, cur, ev, p - 1, wh2);
     if (end_mode == END_MIRROR && p >= 3) {
       // emd_ref.py:92-95: reflect the 2nd / penultimate extremum past each end
-      const Knot<T> a = fetch_knot(S, M, cur, ev, 0, wh);
-      const Knot<T> b = fetch_knot(S, M, cur, ev, 1, wh);
-      int wh2 = NW - 1;
-      const Knot<T> c = fetch_knot(S, M, cur, ev, p - 2, wh2);
-      const Knot<T> d = fetch_knot(S, M, cur, ev, p - 1, wh2);
-      S.vpos[ev][0] = a.t - (b.t - a.t);
-      S.vy[ev][0] = b.y;
-      S.vm[ev][0] = (T)0;
-      S.vpos[ev][MAX_VIRT] = d.t + (d.t - c.t);
-      S.vy[ev][MAX_VIRT] = c.y;
-      S.vm[ev][MAX_VIRT] = (T)0;
+      const T lt = a.t - (b.t - a.t), rt = d.t + (d.t - c.t);
+      S.vt[ev][0] = lt;
+      S.vt[ev][MAX_VIRT] = rt;
+      T* vl = rec(M, cur, ev, M.vbase);
+      T* vr = rec(M, cur, ev, M.vbase + MAX_VIRT);
+      vl[0] = lt; vl[1] = b.y; vl[2] = (T)0;
+      vr[0] = rt; vr[1] = c.y; vr[2] = (T)0;
       S.nl[ev] = 1;
       S.nr[ev] = 1;
       S.P[ev] = p + 2;
     } else {
       // natural: M = 0 at the first and last real knot
-      const Knot<T> a = fetch_knot(S, M, cur, ev, 0, wh);
-      int wh2 = NW - 1;
-      const Knot<T> d = fetch_knot(S, M, cur, ev, p - 1, wh2);
-      M.km[(int64_t)ev * M.rstride + a.code] = (T)0;
-      M.km[(int64_t)ev * M.rstride + d.code] = (T)0;
+      rec(M, cur, ev, a.code)[2] = (T)0;
+      rec(M, cur, ev, d.code)[2] = (T)0;
     }
   }
 }
 
+template <typename T>
+__device__ __forceinline__ int bcode_last_row(const Mem<T>& M, int w0, int w1, int col) {
+  return M.ws[(int64_t)(w1 - w0) * M.cols + col];
+}
+
 template <typename T, int NW>
-__device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int env, int code,
-                                        double val) {
-  if (code >= 0) {
-    M.km[(int64_t)env * M.rstride + code] = (T)val;
+__device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int cur, int env,
+                                        int code, double val) {
+  rec(M, cur, env, code)[2] = (T)val;
+}
+
+// Sequential reader of the knot slots of extended indices e, e+1, ... of
+// (set, env): real knots advance slot by slot inside a segment run and jump
+// to the next non-empty segment at its end; virtual knots come from the
+// VSLOTS records.
+struct KStream {
+  int e;         // extended index of the next knot
+  int w;         // segment of the next real knot
+  int rem;       // real knots left in the current segment run
+  int64_t slot;  // slot of the next real knot
+};
+
+template <typename T, int NW>
+__device__ __forceinline__ void ks_real_at(KStream& ks, const Shared<T, NW>& S, const Mem<T>& M,
+                                           int set, int env, int g) {
+  int w = 0;
+  ks.slot = real_slot(S, M, set, env, g, w);
+  ks.w = w;
+  ks.rem = S.cnt[set][env][w] - (g - S.pre[set][env][w]);
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ void ks_seek(KStream& ks, const Shared<T, NW>& S, const Mem<T>& M,
+                                        int set, int env, int e) {
+  ks.e = e;
+  ks.rem = 0;
+  const int g = e - S.nl[env];
+  if (g >= 0 && g < S.p[env]) ks_real_at(ks, S, M, set, env, g);
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ int64_t ks_next(KStream& ks, const Shared<T, NW>& S, const Mem<T>& M,
+                                           int set, int env) {
+  const int nl = S.nl[env], p = S.p[env];
+  const int g = ks.e - nl;
+  ks.e++;
+  if (g < 0) {
+    if (g == -1 && p > 0) ks_real_at(ks, S, M, set, env, 0);
+    return M.vbase + (g + nl);
+  }
+  if (g >= p) return M.vbase + MAX_VIRT + (g - p);
+  const int64_t s = ks.slot;
+  if (--ks.rem > 0) {
+    ks.slot++;
+  } else if (g + 1 < p) {
+    int w = ks.w + 1;
+#pragma unroll 1
+    while (S.cnt[set][env][w] == 0) ++w;
+    ks.w = w;
+    ks.slot = (int64_t)w * M.cap + S.beg[set][env][w];
+    ks.rem = S.cnt[set][env][w];
+  }
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ void load_ty(const T* r, double& t, double& y) {
+  if constexpr (std::is_same<T, float>::value) {
+    const float2 v = *reinterpret_cast<const float2*>(r);
+    t = v.x;
+    y = v.y;
   } else {
-    const int j = -1 - code;  // < MAX_VIRT: left, else right
-    S.vm[env][j] = (T)val;
+    const double2 v = *reinterpret_cast<const double2*>(r);
+    t = v.x;
+    y = v.y;
   }
 }
 
@@ -629,9 +772,13 @@ __device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int e
 // B rows solved on the clipped window [o0-S, o1+S] with the couplings outside
 // the window dropped (emd_gpu_v5_dsbt.py:51-74, 223-237); one window per
 // thread.  B <= 0: one window per envelope (exact Thomas, emd_ref.py:50-65).
-// Rows are built on the fly from the knots (emd_ref.py:31-48).
+// Rows are built on the fly from the knots (emd_ref.py:31-48).  The serial
+// recurrence is fed by a knot stream read PF rows ahead, and the back
+// substitution reads its forward-sweep rows PF ahead, so memory latency
+// overlaps the division chain.
 template <typename T, int NW>
-__device__ __noinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp) {
+__device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp) {
+  constexpr int PF = 8;
   int nwin[2], nrow[2], Be[2];
   for (int ev = 0; ev < 2; ++ev) {
     const int p = S.p[ev];
@@ -656,51 +803,117 @@ __device__ __noinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, i
     int w1 = o1 + Sp;
     if (w1 > n - 1) w1 = n - 1;
 
-    int wh = 0;
-    Knot<double> ka, kb;
-    {
-      const Knot<T> a = fetch_knot(S, M, cur, ev, w0, wh);
-      const Knot<T> c = fetch_knot(S, M, cur, ev, w0 + 1, wh);
-      ka = {(double)a.t, (double)a.y, 0.0, a.code};
-      kb = {(double)c.t, (double)c.y, 0.0, c.code};
-    }
-    double sl0 = (kb.y - ka.y) / (kb.t - ka.t);
+    KStream ks;
+    ks_seek(ks, S, M, cur, ev, w0);
+    double at, ay, bt, by;
+    load_ty(rec(M, cur, ev, ks_next(ks, S, M, cur, ev)), at, ay);
+    int bcode = (int)ks_next(ks, S, M, cur, ev);
+    load_ty(rec(M, cur, ev, bcode), bt, by);
+    double sl0 = (by - ay) / (bt - at);
     double cprev = 0.0, xprev = 0.0;
-#pragma unroll 1
-    for (int q = w0; q <= w1; ++q) {
-      const Knot<T> cc = fetch_knot(S, M, cur, ev, q + 2, wh);
-      const double ct = (double)cc.t, cy = (double)cc.y;
-      const double H0 = kb.t - ka.t;
-      const double H1 = ct - kb.t;
-      const double d = 2.0 * (H0 + H1);
-      const double l = (q >= 1) ? H0 : 0.0;
-      const double u = (q <= n - 2) ? H1 : 0.0;
-      const double sl1 = (cy - kb.y) / H1;
-      const double r = 6.0 * (sl1 - sl0);
-      const double den = d - l * cprev;
-      const double c = (q == w1) ? 0.0 : u / den;
-      const double xv = (r - l * xprev) / den;
-      const int64_t row = (int64_t)(q - w0) * M.cols + col;
-      M.wc[row] = c;
-      M.wx[row] = xv;
-      M.ws[row] = kb.code;
-      cprev = c;
-      xprev = xv;
-      sl0 = sl1;
-      ka = kb;
-      kb = {ct, cy, 0.0, cc.code};
+
+    const int elast = w1 + 2;  // last knot the window's rows need
+    int enext = w0 + 2;
+    double nt[PF], ny[PF];
+    int nc[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      nc[u] = 0;
+      nt[u] = 0.0;
+      ny[u] = 0.0;
+      if (enext <= elast) {
+        nc[u] = (int)ks_next(ks, S, M, cur, ev);
+        load_ty(rec(M, cur, ev, nc[u]), nt[u], ny[u]);
+        ++enext;
+      }
     }
+#pragma unroll 1
+    for (int q0 = w0; q0 <= w1; q0 += PF) {
+      double ft[PF], fy[PF];
+      int fc[PF];
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        fc[u] = 0;
+        ft[u] = 0.0;
+        fy[u] = 0.0;
+        if (enext <= elast) {
+          fc[u] = (int)ks_next(ks, S, M, cur, ev);
+          load_ty(rec(M, cur, ev, fc[u]), ft[u], fy[u]);
+          ++enext;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int q = q0 + u;
+        if (q <= w1) {
+          const double ct = nt[u], cy = ny[u];
+          const double H0 = bt - at;
+          const double H1 = ct - bt;
+          const double d = 2.0 * (H0 + H1);
+          const double l = (q >= 1) ? H0 : 0.0;
+          const double uu = (q <= n - 2) ? H1 : 0.0;
+          const double sl1 = (cy - by) / H1;
+          const double r = 6.0 * (sl1 - sl0);
+          const double den = d - l * cprev;
+          const double c = (q == w1) ? 0.0 : uu / den;
+          const double xv = (r - l * xprev) / den;
+          const int64_t row = (int64_t)(q - w0) * M.cols + col;
+          M.wcx[row] = make_double2(c, xv);
+          M.ws[row] = bcode;
+          cprev = c;
+          xprev = xv;
+          sl0 = sl1;
+          at = bt;
+          ay = by;
+          bt = ct;
+          by = cy;
+          bcode = nc[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        nt[u] = ft[u];
+        ny[u] = fy[u];
+        nc[u] = fc[u];
+      }
+    }
+    // back substitution, rows w1 .. w0, forward-sweep rows read PF ahead
     double xn = xprev;
-    {
-      const int64_t row = (int64_t)(w1 - w0) * M.cols + col;
-      if (w1 >= o0 && w1 <= o1) store_m(S, M, ev, M.ws[row], xn);
+    if (w1 >= o0 && w1 <= o1) store_m(S, M, cur, ev, bcode_last_row(M, w0, w1, col), xn);
+    double2 cx[PF];
+    int sc[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+      const int q = w1 - 1 - u;
+      const int64_t row = (int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col;
+      cx[u] = M.wcx[row];
+      sc[u] = M.ws[row];
     }
 #pragma unroll 1
-    for (int q = w1 - 1; q >= w0; --q) {
-      const int64_t row = (int64_t)(q - w0) * M.cols + col;
-      const double xi = M.wx[row] - M.wc[row] * xn;
-      if (q >= o0 && q <= o1) store_m(S, M, ev, M.ws[row], xi);
-      xn = xi;
+    for (int q0 = w1 - 1; q0 >= w0; q0 -= PF) {
+      double2 fx[PF];
+      int fs[PF];
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int q = q0 - PF - u;
+        const int64_t row = (int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col;
+        fx[u] = M.wcx[row];
+        fs[u] = M.ws[row];
+      }
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int q = q0 - u;
+        if (q >= w0) {
+          const double xi = cx[u].y - cx[u].x * xn;
+          if (q >= o0 && q <= o1) store_m(S, M, cur, ev, sc[u], xi);
+          xn = xi;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        cx[u] = fx[u];
+        sc[u] = fs[u];
+      }
     }
   }
 }
@@ -708,6 +921,8 @@ __device__ __noinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, i
 // ------------------------------------------------------------------------
 // the persistent kernel
 // ------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t clk() { return wall_clock64(); }
+
 template <typename T, int NT>
 __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   constexpr int NW = NT / WAVE;
@@ -722,16 +937,24 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   M.r = reinterpret_cast<T*>(scr + LY.r);
   M.bits = reinterpret_cast<uint8_t*>(scr + LY.bits);
   M.bstride = A.npad / 8;
-  M.kt = reinterpret_cast<int32_t*>(scr + LY.kt);
-  M.ky = reinterpret_cast<T*>(scr + LY.ky);
-  M.km = reinterpret_cast<T*>(scr + LY.km);
-  M.rstride = (int64_t)NW * A.cap;
-  M.wc = reinterpret_cast<double*>(scr + LY.wc);
-  M.wx = reinterpret_cast<double*>(scr + LY.wx);
+  M.kr = reinterpret_cast<T*>(scr + LY.kr);
+  M.rstride = (int64_t)NW * A.cap + VSLOTS;
+  M.vbase = (int64_t)NW * A.cap;
+  M.wcx = reinterpret_cast<double2*>(scr + LY.wcx);
   M.ws = reinterpret_cast<int32_t*>(scr + LY.ws);
   M.cols = A.cols;
 
   const int trace = A.trace != nullptr;
+  const bool timing = A.phase != nullptr;
+  uint64_t ph[4] = {0, 0, 0, 0};
+  uint64_t tc = timing ? clk() : 0;
+#define PHASE(k)                          \
+  if (timing && tid == 0) {               \
+    const uint64_t t_ = clk();            \
+    ph[k] += t_ - tc;                     \
+    tc = t_;                              \
+  }
+
   for (;;) {
     __syncthreads();
     if (tid == 0) S.task = atomicAdd(A.queue, 1);
@@ -762,12 +985,15 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
         S.hsh[1][0] = S.hsh[1][1] = 0ull;
       }
       __syncthreads();
+      PHASE(3);
       wave_pass<T, NW, PASS_INIT>(S, M, io, 1, 0);
       __syncthreads();
+      PHASE(1);
       fixup(S, M, M.r, 0, trace);
       __syncthreads();
       if (tid == 0) prefix(S, 0);
       __syncthreads();
+      PHASE(2);
 
       int cur = 0, m = 0, total = 0;
       for (;;) {
@@ -795,6 +1021,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           __syncthreads();
           solve(S, M, cur, A.B, A.S);
           __syncthreads();
+          PHASE(0);
           const bool last = (s == A.max_sifts - 1);
           io.src_is_r = (s == 0);
           if (last) {
@@ -810,6 +1037,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
             wave_pass<T, NW, PASS_SIFT>(S, M, io, cur, nxt);
           }
           __syncthreads();
+          PHASE(1);
           fixup(S, M, last ? M.r : M.h, nxt, trace);
           __syncthreads();
           if (tid == 0) {
@@ -822,6 +1050,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
             S.sd = (de > 0.0) ? nu / de : 0.0;
           }
           __syncthreads();
+          PHASE(2);
           ++ns;
           cur = nxt;
           if (last) {
@@ -849,6 +1078,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           if (tid == 0) prefix(S, nxt);
           __syncthreads();
           cur = nxt;
+          PHASE(3);
         }
         __syncthreads();
         if (tid == 0) {
@@ -868,6 +1098,11 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
     __syncthreads();
     for (int j = tid; j < A.max_imf; j += NT) A.acc_cnt[(int64_t)t * A.max_imf + j] = S.acc_cnt[j];
   }
+  if (timing && tid == 0) {
+    PHASE(3);
+    for (int k = 0; k < 4; ++k) A.phase[(int64_t)blockIdx.x * 4 + k] = ph[k];
+  }
+#undef PHASE
 }
 
 // Level-wise ensemble mean and residual (emd_ref.py:209-217): thread per

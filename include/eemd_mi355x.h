@@ -53,6 +53,7 @@ extern "C" {
 
 #define EEMD_FLAG_MEMBER_IMFS 1 /* also return every member's IMFs */
 #define EEMD_FLAG_TRACE 2       /* per-sift extrema counts + index digests */
+#define EEMD_FLAG_PHASES 4      /* per-workgroup phase clocks (eemd_last_phases) */
 
 typedef struct eemd_ctx eemd_ctx;
 
@@ -123,9 +124,18 @@ int eemd_noise(eemd_ctx* ctx, uint64_t seed, int32_t signal, int32_t member0, in
                int64_t n, double* out);
 
 /* Device time (ms, HIP events on the launch stream) of the last call's
- * persistent sift kernel and mean kernel, and its launch configuration. */
-int eemd_last_timing(const eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* threads,
+ * persistent sift kernel and mean kernel, and its launch configuration.
+ * After an asynchronous eemd_run_batch_dev call this waits for its events. */
+int eemd_last_timing(eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* threads,
                      int32_t* grid, int32_t* ntasks);
+
+/* Mean per-workgroup time (ms) of the last call's phases when it ran with
+ * EEMD_FLAG_PHASES: [0] spline setup + block-Thomas solve (+ FP32 interval
+ * coefficients), [1] fused envelope / update / extrema passes, [2] segment
+ * seam fix-up and prefix, [3] noise, finalisation and queue overhead.  No
+ * counterpart in the reference (its phase split used an in-kernel clock,
+ * dbg_serial_split_v0.py:25-42). */
+int eemd_last_phases(eemd_ctx* ctx, double* ms4);
 
 #ifdef __cplusplus
 }
