@@ -114,9 +114,9 @@ class DeviceRun:
         return out
 
 
-def time_mode(wl, fp, steps, warmup, halo):
+def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0):
     import torch
-    r = DeviceRun(wl, fp, halo)
+    r = DeviceRun(wl, fp, halo, threads=threads, mpt=mpt)
     for _ in range(warmup):
         r.step()
     torch.cuda.synchronize()
@@ -185,6 +185,12 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--phases", action="store_true", help="report per-phase kernel time")
+    ap.add_argument("--block", type=int, default=0, help="override the block-Thomas block size")
+    ap.add_argument("--halo32", type=int, default=14)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--mpt", type=int, default=0, help="members per task (0: automatic)")
+    ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
+    ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the EEMD engine runs on a single GPU here "
@@ -192,7 +198,13 @@ def main():
         sys.exit(2)
 
     wl = workload(a.workload)
-    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=14)
+    if a.block:
+        wl["B"] = a.block
+    if a.E:
+        wl["E"] = a.E
+    if a.max_imf:
+        wl["max_imf"] = a.max_imf
+    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads, mpt=a.mpt)
     out = {
         "metric": "ensemble sample-sifts/s",
         "value": f32["value"],
@@ -207,7 +219,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": wl["label"], "fp": "f32 (spline solve and ensemble sum in f64)",
-                   "halo": 14, "threads": f32["threads"], "grid": f32["grid"],
+                   "halo": a.halo32, "block": wl["B"], "threads": f32["threads"], "grid": f32["grid"],
                    "tasks": f32["ntasks"], "noise": "philox in-kernel",
                    "sample_sifts_per_step": f32["sample_sifts"]},
         "roofline": f32["roofline"],
@@ -218,7 +230,8 @@ def main():
         r32.step(phases=True)
         out["phases_ms_f32"] = dict(zip(("solve", "pass", "fixup", "other"), r32.eng.last_phases()))
     if not a.no_fp64:
-        f64, r64 = time_mode(wl, "f64", max(1, a.steps // 2), a.warmup, halo=30)
+        f64, r64 = time_mode(wl, "f64", max(1, a.steps // 2), a.warmup, halo=30,
+                             threads=a.threads, mpt=a.mpt)
         out["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "sift_kernel_ms", "roofline")}
         # FP32 error per ensemble-mean IMF against the FP64 engine (same noise)
         i32, i64 = r32.imfs(), r64.imfs()

@@ -289,35 +289,6 @@ __device__ __forceinline__ V sel(int j, V a0, V a1, V a2, V a3, V a4, V a5) {
   return r;
 }
 
-// FP64: values of one envelope at samples i0..i0+7; the lane's knots
-// e0..e0+NC-1 are loaded once and selected per sample.
-template <int NC, int NW>
-__device__ __forceinline__ void env_knots(const Shared<double, NW>& S, const Mem<double>& M,
-                                          int cur, int ev, int w, const int (&e)[RPL], int i0,
-                                          double (&out)[RPL]) {
-  const int P = S.P[ev];
-  int wh = w;
-  auto ld = [&](int c) {
-    int ec = e[0] + c;
-    ec = ec > P - 1 ? P - 1 : ec;
-    return knot_at(S, M, cur, ev, ec, wh);
-  };
-  const Knot<double> k0 = ld(0), k1 = ld(1), k2 = ld(NC > 2 ? 2 : 1);
-  const Knot<double> k3 = ld(NC > 3 ? 3 : 1), k4 = ld(NC > 4 ? 4 : 1), k5 = ld(NC > 5 ? 5 : 1);
-#pragma unroll
-  for (int k = 0; k < RPL; ++k) {
-    const int j = e[k] - e[0];
-    const int j1 = j + 1;
-    const double ta = sel<NC>(j, k0.t, k1.t, k2.t, k3.t, k4.t, k5.t);
-    const double ya = sel<NC>(j, k0.y, k1.y, k2.y, k3.y, k4.y, k5.y);
-    const double ma = sel<NC>(j, k0.m, k1.m, k2.m, k3.m, k4.m, k5.m);
-    const double tb = sel<NC>(j1, k0.t, k1.t, k2.t, k3.t, k4.t, k5.t);
-    const double yb = sel<NC>(j1, k0.y, k1.y, k2.y, k3.y, k4.y, k5.y);
-    const double mb = sel<NC>(j1, k0.m, k1.m, k2.m, k3.m, k4.m, k5.m);
-    out[k] = spline_val(ta, ya, ma, tb, yb, mb, (double)(i0 + k));
-  }
-}
-
 // FP32: values of one envelope at samples i0..i0+7.  The lane loads its
 // knots e0..e0+NC, turns each interval into Horner form (the expansion of
 // emd_ref.py:71-75 in tau = t - t0: c1 = (y1-y0)/h - h(2 M0 + M1)/6,
@@ -328,7 +299,7 @@ struct Ivl4 {
 
 __device__ __forceinline__ Ivl4 make_ivl(const Knot<float>& a, const Knot<float>& b) {
   const float hh = b.t - a.t;
-  const float ih = 1.0f / hh;
+  const float ih = __builtin_amdgcn_rcpf(hh);
   Ivl4 v;
   v.t = a.t;
   v.y = a.y;
@@ -336,70 +307,6 @@ __device__ __forceinline__ Ivl4 make_ivl(const Knot<float>& a, const Knot<float>
   v.c2 = 0.5f * a.m;
   v.c3 = (b.m - a.m) * ih * (1.0f / 6.0f);
   return v;
-}
-
-template <int NC, int NW>
-__device__ __forceinline__ void env_ivls(const Shared<float, NW>& S, const Mem<float>& M, int cur,
-                                         int ev, int w, const int (&e)[RPL], int i0,
-                                         float (&out)[RPL]) {
-  const int P = S.P[ev];
-  int wh = w;
-  auto ld = [&](int c) {
-    int ec = e[0] + c;
-    ec = ec > P - 1 ? P - 1 : ec;
-    return knot_at(S, M, cur, ev, ec, wh);
-  };
-  const Knot<float> k0 = ld(0), k1 = ld(1);
-  const Knot<float> k2 = ld(NC > 1 ? 2 : 1), k3 = ld(NC > 2 ? 3 : 1);
-  const Knot<float> k4 = ld(NC > 3 ? 4 : 1), k5 = ld(NC > 4 ? 5 : 1);
-  const Ivl4 v0 = make_ivl(k0, k1);
-  const Ivl4 v1 = NC > 1 ? make_ivl(k1, k2) : v0;
-  const Ivl4 v2 = NC > 2 ? make_ivl(k2, k3) : v0;
-  const Ivl4 v3 = NC > 3 ? make_ivl(k3, k4) : v0;
-  const Ivl4 v4 = NC > 4 ? make_ivl(k4, k5) : v0;
-#pragma unroll
-  for (int k = 0; k < RPL; ++k) {
-    const int j = e[k] - e[0];
-    const float t = sel<NC>(j, v0.t, v1.t, v2.t, v3.t, v4.t, v4.t);
-    const float y = sel<NC>(j, v0.y, v1.y, v2.y, v3.y, v4.y, v4.y);
-    const float c1 = sel<NC>(j, v0.c1, v1.c1, v2.c1, v3.c1, v4.c1, v4.c1);
-    const float c2 = sel<NC>(j, v0.c2, v1.c2, v2.c2, v3.c2, v4.c2, v4.c2);
-    const float c3 = sel<NC>(j, v0.c3, v1.c3, v2.c3, v3.c3, v4.c3, v4.c3);
-    const float tau = (float)(i0 + k) - t;
-    out[k] = __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, c3, c2), c1), y);
-  }
-}
-
-// Values of envelope `ev` (set cur) at the lane's samples i0..i0+7.
-// base = number of real knots with bit index < i0; byte = the lane's bits.
-template <typename T, int NW>
-__device__ __forceinline__ void envelope8(const Shared<T, NW>& S, const Mem<T>& M, int cur, int ev,
-                                          int w, int base, int byte, int i0, T (&out)[RPL]) {
-  const int p = S.p[ev];
-  if (p < 2) {
-#pragma unroll
-    for (int k = 0; k < RPL; ++k) out[k] = S.cval[ev];
-    return;
-  }
-  const int nl = S.nl[ev], nr = S.nr[ev], P = S.P[ev];
-  int e[RPL];
-#pragma unroll
-  for (int k = 0; k < RPL; ++k)
-    e[k] = ext_interval(S, ev, base + __popc(byte & ((2u << k) - 1u)), i0 + k, nl, nr, p, P);
-  const int span = e[RPL - 1] - e[0];
-  // wave-uniform number of candidates: strict extrema of one kind are >= 2
-  // samples apart, so 8 samples meet at most 5 intervals
-  const bool need3 = __ballot(span >= 2) != 0ull;
-  const bool need5 = __ballot(span >= 3) != 0ull;
-  if constexpr (std::is_same<T, double>::value) {
-    if (need5) env_knots<6>(S, M, cur, ev, w, e, i0, out);
-    else if (need3) env_knots<4>(S, M, cur, ev, w, e, i0, out);
-    else env_knots<3>(S, M, cur, ev, w, e, i0, out);
-  } else {
-    if (need5) env_ivls<5>(S, M, cur, ev, w, e, i0, out);
-    else if (need3) env_ivls<3>(S, M, cur, ev, w, e, i0, out);
-    else env_ivls<2>(S, M, cur, ev, w, e, i0, out);
-  }
 }
 
 // ------------------------------------------------------------------------
@@ -421,6 +328,180 @@ struct PassIO {
   int trace;            // accumulate knot digests
 };
 
+// ------------------------------------------------------------------------
+// software-pipelined sift pass
+// ------------------------------------------------------------------------
+// Per envelope and chunk: the lane's 8 extrema bits of the current set, the
+// number of real knots before its first sample, the extended index of its
+// first interval and up to six knots from there (loaded one chunk ahead).
+template <typename T>
+struct EnvChunk {
+  int byte, base, e0;
+  int fast;  // wave-uniform: every sample of the chunk lies between two real knots
+  T t0, y0, m0, t1, y1, m1, t2, y2, m2, t3, y3, m3, t4, y4, m4, t5, y5, m5;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_tym(const T* r, T& t, T& y, T& m) {
+  if constexpr (std::is_same<T, float>::value) {
+    const float4 v = *reinterpret_cast<const float4*>(r);
+    t = v.x;
+    y = v.y;
+    m = v.z;
+  } else {
+    const double2 v = *reinterpret_cast<const double2*>(r);
+    t = v.x;
+    y = v.y;
+    m = r[2];
+  }
+}
+
+// Stage 2 of the pipeline for one envelope: interval indices of the chunk
+// and the loads of its knots.  Returns the wave-uniform candidate class
+// (0: <= 2 intervals per lane, 1: <= 3, 2: <= 5).  Interior chunks (every
+// sample between two real knots, decided for the whole wave from lanes 0 and
+// 63) take the fast path: interval index = popcount of the bits, knot slots
+// consecutive inside the segment run.
+template <typename T, int NW>
+__device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>& M, int cur,
+                                            int ev, int w, int byte, int& crun, int i0,
+                                            EnvChunk<T>& c) {
+  int tot;
+  const int ex = wave_excl_small(__popc(byte), tot);
+  c.byte = byte;
+  c.base = S.pre[cur][ev][w] + crun + ex;
+  crun += tot;
+  c.fast = 0;
+  const int p = S.p[ev];
+  if (p < 2) {
+    c.e0 = 0;
+    return 0;
+  }
+  const int nl = S.nl[ev];
+  const int cl0 = c.base + (byte & 1);
+  const int cl7 = c.base + __popc(byte);
+  const int lo = __shfl(cl0, 0);
+  const int hi = __shfl(cl7, WAVE - 1);
+  T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
+  if (lo >= 1 && hi <= p - 1) {
+    c.fast = 1;
+    c.e0 = cl0 + nl - 1;
+    const int span = cl7 - cl0;
+    const int cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
+    const int g0 = cl0 - 1;
+    int wg = w;
+    const int64_t s0 = real_slot(S, M, cur, ev, g0, wg);
+    const int rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+    const int need = span + 2;  // knots e0 .. e0+span+1, all real
+    if (rem >= need) {
+      const T* r = rb + s0 * rec_len<T>();
+      load_tym(r, c.t0, c.y0, c.m0);
+      load_tym(r + rec_len<T>(), c.t1, c.y1, c.m1);
+      if (need > 2) load_tym(r + 2 * rec_len<T>(), c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) load_tym(r + 3 * rec_len<T>(), c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) load_tym(r + 4 * rec_len<T>(), c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) load_tym(r + 5 * rec_len<T>(), c.t5, c.y5, c.m5);
+    } else {
+      int wh = wg;
+      auto at = [&](int k) { return rb + real_slot(S, M, cur, ev, g0 + k, wh) * rec_len<T>(); };
+      load_tym(at(0), c.t0, c.y0, c.m0);
+      load_tym(at(1), c.t1, c.y1, c.m1);
+      if (need > 2) load_tym(at(2), c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) load_tym(at(3), c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) load_tym(at(4), c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) load_tym(at(5), c.t5, c.y5, c.m5);
+    }
+    return cls;
+  }
+  const int nr = S.nr[ev], P = S.P[ev];
+  c.e0 = ext_interval(S, ev, cl0, i0, nl, nr, p, P);
+  const int e7 = ext_interval(S, ev, cl7, i0 + RPL - 1, nl, nr, p, P);
+  const int span = e7 - c.e0;
+  const int cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
+  int wh = w;
+  const int last = P - 1;
+  auto at = [&](int k) {
+    int e = c.e0 + k;
+    e = e > last ? last : e;
+    return rb + ext_slot(S, M, cur, ev, e, wh) * rec_len<T>();
+  };
+  load_tym(at(0), c.t0, c.y0, c.m0);
+  load_tym(at(1), c.t1, c.y1, c.m1);
+  load_tym(at(2), c.t2, c.y2, c.m2);
+  if (cls >= 1) load_tym(at(3), c.t3, c.y3, c.m3);
+  if (cls >= 2) {
+    load_tym(at(4), c.t4, c.y4, c.m4);
+    load_tym(at(5), c.t5, c.y5, c.m5);
+  }
+  return cls;
+}
+
+// Stage 3: values of the envelope at samples i0..i0+7 (written for ev 0,
+// added for ev 1).
+template <int NC, bool FAST, typename T, int NW>
+__device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const EnvChunk<T>& c,
+                                         int i0, T (&acc)[RPL]) {
+  const int nl = S.nl[ev], nr = S.nr[ev], p = S.p[ev], P = S.P[ev];
+  if constexpr (std::is_same<T, double>::value) {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int j = FAST ? __popc(c.byte & ((2u << k) - 2u))
+                         : ext_interval(S, ev, c.base + __popc(c.byte & ((2u << k) - 1u)), i0 + k,
+                                        nl, nr, p, P) - c.e0;
+      const int j1 = j + 1;
+      const double v = spline_val(sel<NC + 1>(j, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5),
+                                  sel<NC + 1>(j, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5),
+                                  sel<NC + 1>(j, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5),
+                                  sel<NC + 1>(j1, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5),
+                                  sel<NC + 1>(j1, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5),
+                                  sel<NC + 1>(j1, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5),
+                                  (double)(i0 + k));
+      acc[k] = ev == 0 ? v : acc[k] + v;
+    }
+  } else {
+    const Knot<float> k0{c.t0, c.y0, c.m0, 0}, k1{c.t1, c.y1, c.m1, 0}, k2{c.t2, c.y2, c.m2, 0};
+    const Knot<float> k3{c.t3, c.y3, c.m3, 0}, k4{c.t4, c.y4, c.m4, 0}, k5{c.t5, c.y5, c.m5, 0};
+    const Ivl4 v0 = make_ivl(k0, k1);
+    const Ivl4 v1 = make_ivl(k1, k2);
+    const Ivl4 v2 = NC > 2 ? make_ivl(k2, k3) : v1;
+    const Ivl4 v3 = NC > 3 ? make_ivl(k3, k4) : v1;
+    const Ivl4 v4 = NC > 4 ? make_ivl(k4, k5) : v1;
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const int j = FAST ? __popc(c.byte & ((2u << k) - 2u))
+                         : ext_interval(S, ev, c.base + __popc(c.byte & ((2u << k) - 1u)), i0 + k,
+                                        nl, nr, p, P) - c.e0;
+      const float t = sel<NC>(j, v0.t, v1.t, v2.t, v3.t, v4.t, v4.t);
+      const float y = sel<NC>(j, v0.y, v1.y, v2.y, v3.y, v4.y, v4.y);
+      const float c1 = sel<NC>(j, v0.c1, v1.c1, v2.c1, v3.c1, v4.c1, v4.c1);
+      const float c2 = sel<NC>(j, v0.c2, v1.c2, v2.c2, v3.c2, v4.c2, v4.c2);
+      const float c3 = sel<NC>(j, v0.c3, v1.c3, v2.c3, v3.c3, v4.c3, v4.c3);
+      const float tau = (float)(i0 + k) - t;
+      const float v = __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, c3, c2), c1), y);
+      acc[k] = ev == 0 ? v : acc[k] + v;
+    }
+  }
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ void env_eval_any(const Shared<T, NW>& S, int ev, int cls,
+                                             const EnvChunk<T>& c, int i0, T (&acc)[RPL]) {
+  if (S.p[ev] < 2) {
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) acc[k] = ev == 0 ? S.cval[ev] : acc[k] + S.cval[ev];
+    return;
+  }
+  if (c.fast) {
+    if (cls == 2) env_eval<5, true>(S, ev, c, i0, acc);
+    else if (cls == 1) env_eval<3, true>(S, ev, c, i0, acc);
+    else env_eval<2, true>(S, ev, c, i0, acc);
+  } else {
+    if (cls == 2) env_eval<5, false>(S, ev, c, i0, acc);
+    else if (cls == 1) env_eval<3, false>(S, ev, c, i0, acc);
+    else env_eval<2, false>(S, ev, c, i0, acc);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void put_knot(T* r, int q, T val) {
   if constexpr (std::is_same<T, float>::value)
@@ -429,36 +510,221 @@ __device__ __forceinline__ void put_knot(T* r, int q, T val) {
     *reinterpret_cast<double2*>(r) = make_double2((double)q, val);
 }
 
-// One wave streams its segment: produce new values, write them, detect the
-// extrema of the new values and compact them into set `nxt`.
-template <typename T, int NW, int MODE>
-__device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, int cur, int nxt) {
+// Extrema of the new values v (bit b <=> position b-1), written to the
+// bitmaps of set nxt and compacted, in time order, into the wave's segment
+// region (emd_ref.py:84-87).  p1/p2 come from the lane before (or the carry
+// of the previous chunk); the first two positions of a segment are decided
+// by fixup() after the pass.
+template <typename T, int NW>
+__device__ __forceinline__ void detect_compact(const Mem<T>& M, int nxt, int64_t cb, int64_t seg_lo,
+                                               const T (&v)[RPL], T& carry1, T& carry2,
+                                               int (&run)[2], unsigned long long (&hs)[2],
+                                               int trace, int64_t seg_slot) {
+  const int ln = lane_id();
+  const int64_t n = M.n;
+  const int64_t i0 = cb + (int64_t)ln * RPL;
+  const int kcap = (int)(M.cap - SEG_FRONT);
+  uint8_t* const bn0 = M.bits + (int64_t)(nxt * 2 + 0) * M.bstride;
+  uint8_t* const bn1 = M.bits + (int64_t)(nxt * 2 + 1) * M.bstride;
+  T p1 = __shfl_up(v[RPL - 1], 1);
+  T p2 = __shfl_up(v[RPL - 2], 1);
+  if (ln == 0) {
+    p1 = carry1;
+    p2 = carry2;
+  }
+  carry1 = __shfl(v[RPL - 1], WAVE - 1);
+  carry2 = __shfl(v[RPL - 2], WAVE - 1);
+  const bool first = (cb == seg_lo) && (ln == 0);
+  int bmax = 0, bmin = 0;
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    const int64_t b = i0 + k;
+    const T vm1 = (k == 0) ? p2 : ((k == 1) ? p1 : v[k - 2]);
+    const T v0 = (k == 0) ? p1 : v[k - 1];
+    const T vp1 = v[k];
+    const bool ok = (b >= 2) && (b <= n - 1) && !(first && k < 2);
+    if (ok && v0 > vm1 && v0 > vp1) bmax |= 1 << k;
+    if (ok && v0 < vm1 && v0 < vp1) bmin |= 1 << k;
+  }
+  bn0[cb / 8 + ln] = (uint8_t)bmax;
+  bn1[cb / 8 + ln] = (uint8_t)bmin;
+#pragma unroll
+  for (int ev = 0; ev < 2; ++ev) {
+    const int bits = ev == 0 ? bmax : bmin;
+    int tot;
+    const int ex = wave_excl_small(__popc(bits), tot);
+    int o = run[ev] + ex;
+    if (bits) {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        if ((bits >> k) & 1) {
+          const int q = (int)(i0 + k - 1);
+          const T val = (k == 0) ? p1 : v[k - 1];
+          if (o < kcap) put_knot(rec(M, nxt, ev, seg_slot + o), q, val);
+          if (trace) hs[ev] += mix64((unsigned long long)q);
+          ++o;
+        }
+      }
+    }
+    run[ev] += tot;
+  }
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ void pass_epilogue(Shared<T, NW>& S, int nxt, int w,
+                                              const int (&run)[2],
+                                              const unsigned long long (&hs)[2], int trace,
+                                              bool want_sd, double num, double den) {
+  const int ln = lane_id();
+  if (trace) {
+    const unsigned long long a = wave_sum_u64(hs[0]);
+    const unsigned long long b = wave_sum_u64(hs[1]);
+    if (ln == 0) {
+      atomicAdd(&S.hsh[nxt][0], a);
+      atomicAdd(&S.hsh[nxt][1], b);
+    }
+  }
+  if (want_sd) {
+    num = wave_sum(num);
+    den = wave_sum(den);
+  }
+  if (ln == 0) {
+    S.cnt[nxt][0][w] = run[0];
+    S.cnt[nxt][1][w] = run[1];
+    S.beg[nxt][0][w] = SEG_FRONT;
+    S.beg[nxt][1][w] = SEG_FRONT;
+    S.red[0][w] = num;
+    S.red[1][w] = den;
+  }
+}
+
+// SIFT / LAST pass, software-pipelined: while chunk c is evaluated, the
+// bitmap bytes of chunk c+2 and the knots, h (and r) of chunk c+1 are in
+// flight.
+template <typename T, int NW, bool LAST>
+__device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io,
+                                               int cur, int nxt) {
   const int ln = lane_id();
   const int w = (int)(threadIdx.x / WAVE);
   const int64_t n = M.n;
   const int64_t seg_lo = (int64_t)w * M.sl;
   int64_t seg_hi = seg_lo + M.sl;
   if (seg_hi > n) seg_hi = n;
-
-  int run[2] = {0, 0};      // knots of set nxt written in this segment
-  int crun[2] = {0, 0};     // knots of set cur passed in this segment (eval)
+  int run[2] = {0, 0};
   unsigned long long hs[2] = {0ull, 0ull};
   double num = 0.0, den = 0.0;
   T carry1 = (T)0, carry2 = (T)0;
-
-  T* const dst = (MODE == PASS_SIFT) ? M.h : M.r;
+  const bool want_sd = !LAST && io.want_sd;
   const T* const src = io.src_is_r ? M.r : M.h;
-  uint8_t* const bn0 = M.bits + (int64_t)(nxt * 2 + 0) * M.bstride;
-  uint8_t* const bn1 = M.bits + (int64_t)(nxt * 2 + 1) * M.bstride;
   const uint8_t* const bc0 = M.bits + (int64_t)(cur * 2 + 0) * M.bstride;
   const uint8_t* const bc1 = M.bits + (int64_t)(cur * 2 + 1) * M.bstride;
-  const int kcap = (int)(M.cap - SEG_FRONT);
   const int64_t seg_slot = (int64_t)w * M.cap + SEG_FRONT;
+
+  if (seg_lo < seg_hi) {
+    int crun0 = 0, crun1 = 0;
+    int64_t cb = seg_lo;
+    const int64_t i00 = cb + (int64_t)ln * RPL;
+    int fb0 = 0, fb1 = 0;
+    if (cb + CHUNK < seg_hi) {
+      fb0 = bc0[(cb + CHUNK) / 8 + ln];
+      fb1 = bc1[(cb + CHUNK) / 8 + ln];
+    }
+    EnvChunk<T> A0, A1;
+    int cls0 = env_prefetch(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
+    int cls1 = env_prefetch(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+    T hv[RPL], rv[RPL];
+    load8(src + i00, hv);
+    if constexpr (LAST) load8(M.r + i00, rv);
+#pragma unroll 1
+    for (; cb < seg_hi; cb += CHUNK) {
+      const int64_t i0 = cb + (int64_t)ln * RPL;
+      const bool has1 = cb + CHUNK < seg_hi;
+      const bool has2 = cb + 2 * CHUNK < seg_hi;
+      int gb0 = 0, gb1 = 0;
+      if (has2) {
+        gb0 = bc0[(cb + 2 * CHUNK) / 8 + ln];
+        gb1 = bc1[(cb + 2 * CHUNK) / 8 + ln];
+      }
+      EnvChunk<T> B0, B1;
+      int clsB0 = 0, clsB1 = 0;
+      T hn[RPL], rn[RPL];
+      if (has1) {
+        clsB0 = env_prefetch(S, M, cur, 0, w, fb0, crun0, (int)(i0 + CHUNK), B0);
+        clsB1 = env_prefetch(S, M, cur, 1, w, fb1, crun1, (int)(i0 + CHUNK), B1);
+        load8(src + i0 + CHUNK, hn);
+        if constexpr (LAST) load8(M.r + i0 + CHUNK, rn);
+      }
+      T msum[RPL];
+      env_eval_any(S, 0, cls0, A0, (int)i0, msum);
+      env_eval_any(S, 1, cls1, A1, (int)i0, msum);
+      T v[RPL];
+      if constexpr (!LAST) {
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          v[k] = hv[k] - (T)0.5 * msum[k];
+          if (want_sd && i0 + k < n) {
+            const T dd = hv[k] - v[k];
+            num += (double)(dd * dd);
+            den += (double)(hv[k] * hv[k]);
+          }
+        }
+        store8(M.h + i0, v);
+      } else {
+        T imf[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          imf[k] = hv[k] - (T)0.5 * msum[k];
+          v[k] = rv[k] - imf[k];
+        }
+        store8(M.r + i0, v);
+        if (io.imf) {
+          T* o = reinterpret_cast<T*>(io.imf);
+#pragma unroll
+          for (int k = 0; k < RPL; ++k) if (i0 + k < n) o[i0 + k] = imf[k];
+        }
+        if (io.acc) {
+#pragma unroll
+          for (int k = 0; k < RPL; ++k)
+            if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)imf[k];
+        }
+      }
+      detect_compact<T, NW>(M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace, seg_slot);
+      A0 = B0;
+      A1 = B1;
+      cls0 = clsB0;
+      cls1 = clsB1;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        hv[k] = hn[k];
+        if constexpr (LAST) rv[k] = rn[k];
+      }
+      fb0 = gb0;
+      fb1 = gb1;
+    }
+  }
+  pass_epilogue(S, nxt, w, run, hs, io.trace, want_sd, num, den);
+}
+
+// INIT / FINAL pass: one wave streams its segment, produces the new residual
+// (noise added, or the IMF subtracted), writes it and compacts its extrema
+// into set `nxt`.
+template <typename T, int NW, int MODE>
+__device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io, int cur, int nxt) {
+  static_assert(MODE == PASS_INIT || MODE == PASS_FINAL, "sift passes use wave_sift_pipe");
+  const int ln = lane_id();
+  const int w = (int)(threadIdx.x / WAVE);
+  const int64_t n = M.n;
+  const int64_t seg_lo = (int64_t)w * M.sl;
+  int64_t seg_hi = seg_lo + M.sl;
+  if (seg_hi > n) seg_hi = n;
+  int run[2] = {0, 0};
+  unsigned long long hs[2] = {0ull, 0ull};
+  T carry1 = (T)0, carry2 = (T)0;
+  const T* const src = io.src_is_r ? M.r : M.h;
 
   for (int64_t cb = seg_lo; cb < seg_hi; cb += CHUNK) {
     const int64_t i0 = cb + (int64_t)ln * RPL;
     T v[RPL];
-
     if constexpr (MODE == PASS_INIT) {
       double zz[RPL];
       if (io.z) {
@@ -474,8 +740,8 @@ __device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, con
         const double xv = (i0 + k < n) ? io.x[i0 + k] : 0.0;
         v[k] = (T)(xv + zz[k] * io.sigma);
       }
-      store8(dst + i0, v);
-    } else if constexpr (MODE == PASS_FINAL) {
+      store8(M.r + i0, v);
+    } else {
       T hv[RPL];
       load8(src + i0, hv);
       load8(M.r + i0, v);
@@ -492,120 +758,11 @@ __device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, con
         for (int k = 0; k < RPL; ++k)
           if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)hv[k];
       }
-    } else {
-      // PASS_SIFT / PASS_LAST: mean envelope, then h (and r)
-      T msum[RPL], e1[RPL];
-      {
-        const int b0 = bc0[cb / 8 + ln];
-        const int b1 = bc1[cb / 8 + ln];
-        int t0, t1;
-        const int x0 = wave_excl_small(__popc(b0), t0);
-        const int x1 = wave_excl_small(__popc(b1), t1);
-        const int base0 = S.pre[cur][0][w] + crun[0] + x0;
-        const int base1 = S.pre[cur][1][w] + crun[1] + x1;
-        crun[0] += t0;
-        crun[1] += t1;
-        envelope8(S, M, cur, 0, w, base0, b0, (int)i0, msum);
-        envelope8(S, M, cur, 1, w, base1, b1, (int)i0, e1);
-      }
-      T hv[RPL];
-      load8(src + i0, hv);
-      if constexpr (MODE == PASS_SIFT) {
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-          v[k] = hv[k] - (T)0.5 * (msum[k] + e1[k]);
-          if (io.want_sd && i0 + k < n) {
-            const T dd = hv[k] - v[k];
-            num += (double)(dd * dd);
-            den += (double)(hv[k] * hv[k]);
-          }
-        }
-        store8(M.h + i0, v);
-      } else {
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) hv[k] = hv[k] - (T)0.5 * (msum[k] + e1[k]);  // the IMF
-        load8(M.r + i0, v);
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) v[k] = v[k] - hv[k];                          // new residual
-        store8(M.r + i0, v);
-        if (io.imf) {
-          T* o = reinterpret_cast<T*>(io.imf);
-#pragma unroll
-          for (int k = 0; k < RPL; ++k) if (i0 + k < n) o[i0 + k] = hv[k];
-        }
-        if (io.acc) {
-#pragma unroll
-          for (int k = 0; k < RPL; ++k)
-            if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)hv[k];
-        }
-      }
     }
-
-    // ---- extrema of the new values (bit b <=> position b-1) ----
-    T p1 = __shfl_up(v[RPL - 1], 1);
-    T p2 = __shfl_up(v[RPL - 2], 1);
-    if (ln == 0) {
-      p1 = carry1;
-      p2 = carry2;
-    }
-    carry1 = __shfl(v[RPL - 1], WAVE - 1);
-    carry2 = __shfl(v[RPL - 2], WAVE - 1);
-    const bool first = (cb == seg_lo) && (ln == 0);
-    int bmax = 0, bmin = 0;
-#pragma unroll
-    for (int k = 0; k < RPL; ++k) {
-      const int64_t b = i0 + k;
-      const T vm1 = (k == 0) ? p2 : ((k == 1) ? p1 : v[k - 2]);
-      const T v0 = (k == 0) ? p1 : v[k - 1];
-      const T vp1 = v[k];
-      const bool ok = (b >= 2) && (b <= n - 1) && !(first && k < 2);
-      if (ok && v0 > vm1 && v0 > vp1) bmax |= 1 << k;
-      if (ok && v0 < vm1 && v0 < vp1) bmin |= 1 << k;
-    }
-    bn0[cb / 8 + ln] = (uint8_t)bmax;
-    bn1[cb / 8 + ln] = (uint8_t)bmin;
-#pragma unroll
-    for (int ev = 0; ev < 2; ++ev) {
-      const int bits = ev == 0 ? bmax : bmin;
-      int tot;
-      const int ex = wave_excl_small(__popc(bits), tot);
-      int o = run[ev] + ex;
-      if (bits) {
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-          if ((bits >> k) & 1) {
-            const int q = (int)(i0 + k - 1);
-            const T val = (k == 0) ? p1 : v[k - 1];
-            if (o < kcap) put_knot(rec(M, nxt, ev, seg_slot + o), q, val);
-            if (io.trace) hs[ev] += mix64((unsigned long long)q);
-            ++o;
-          }
-        }
-      }
-      run[ev] += tot;
-    }
+    detect_compact<T, NW>(M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace,
+                          (int64_t)w * M.cap + SEG_FRONT);
   }
-
-  if (io.trace) {
-    const unsigned long long a = wave_sum_u64(hs[0]);
-    const unsigned long long b = wave_sum_u64(hs[1]);
-    if (ln == 0) {
-      atomicAdd(&S.hsh[nxt][0], a);
-      atomicAdd(&S.hsh[nxt][1], b);
-    }
-  }
-  if (MODE == PASS_SIFT && io.want_sd) {
-    num = wave_sum(num);
-    den = wave_sum(den);
-  }
-  if (ln == 0) {
-    S.cnt[nxt][0][w] = run[0];
-    S.cnt[nxt][1][w] = run[1];
-    S.beg[nxt][0][w] = SEG_FRONT;
-    S.beg[nxt][1][w] = SEG_FRONT;
-    S.red[0][w] = num;
-    S.red[1][w] = den;
-  }
+  pass_epilogue(S, nxt, w, run, hs, io.trace, false, 0.0, 0.0);
 }
 
 // Positions seg_lo-1 and seg_lo of each segment need values of the previous
@@ -814,39 +971,44 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
 
     const int elast = w1 + 2;  // last knot the window's rows need
     int enext = w0 + 2;
-    double nt[PF], ny[PF];
+    // prefetched knots stay in their storage type until used, so the loads
+    // are not waited for at issue
+    using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+    T2 nk[PF];
     int nc[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       nc[u] = 0;
-      nt[u] = 0.0;
-      ny[u] = 0.0;
       if (enext <= elast) {
         nc[u] = (int)ks_next(ks, S, M, cur, ev);
-        load_ty(rec(M, cur, ev, nc[u]), nt[u], ny[u]);
+        nk[u] = *reinterpret_cast<const T2*>(rec(M, cur, ev, nc[u]));
         ++enext;
+      } else {
+        nk[u].x = (T)0;
+        nk[u].y = (T)0;
       }
     }
 #pragma unroll 1
     for (int q0 = w0; q0 <= w1; q0 += PF) {
-      double ft[PF], fy[PF];
+      T2 fk[PF];
       int fc[PF];
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         fc[u] = 0;
-        ft[u] = 0.0;
-        fy[u] = 0.0;
         if (enext <= elast) {
           fc[u] = (int)ks_next(ks, S, M, cur, ev);
-          load_ty(rec(M, cur, ev, fc[u]), ft[u], fy[u]);
+          fk[u] = *reinterpret_cast<const T2*>(rec(M, cur, ev, fc[u]));
           ++enext;
+        } else {
+          fk[u].x = (T)0;
+          fk[u].y = (T)0;
         }
       }
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int q = q0 + u;
         if (q <= w1) {
-          const double ct = nt[u], cy = ny[u];
+          const double ct = (double)nk[u].x, cy = (double)nk[u].y;
           const double H0 = bt - at;
           const double H1 = ct - bt;
           const double d = 2.0 * (H0 + H1);
@@ -872,8 +1034,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
       }
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
-        nt[u] = ft[u];
-        ny[u] = fy[u];
+        nk[u] = fk[u];
         nc[u] = fc[u];
       }
     }
@@ -1019,6 +1180,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
             }
           }
           __syncthreads();
+          PHASE(3);
           solve(S, M, cur, A.B, A.S);
           __syncthreads();
           PHASE(0);
@@ -1031,10 +1193,10 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
                          : nullptr;
             io.acc = A.acc ? A.acc + td.acc_off + (int64_t)m * sd.n : nullptr;
             io.acc_first = (S.acc_cnt[m] == 0);
-            wave_pass<T, NW, PASS_LAST>(S, M, io, cur, nxt);
+            wave_sift_pipe<T, NW, true>(S, M, io, cur, nxt);
           } else {
             io.want_sd = (A.tau > 0.0) && (s >= 1);
-            wave_pass<T, NW, PASS_SIFT>(S, M, io, cur, nxt);
+            wave_sift_pipe<T, NW, false>(S, M, io, cur, nxt);
           }
           __syncthreads();
           PHASE(1);

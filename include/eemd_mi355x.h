@@ -130,9 +130,9 @@ int eemd_last_timing(eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* t
                      int32_t* grid, int32_t* ntasks);
 
 /* Mean per-workgroup time (ms) of the last call's phases when it ran with
- * EEMD_FLAG_PHASES: [0] spline setup + block-Thomas solve (+ FP32 interval
- * coefficients), [1] fused envelope / update / extrema passes, [2] segment
- * seam fix-up and prefix, [3] noise, finalisation and queue overhead.  No
+ * EEMD_FLAG_PHASES: [0] block-Thomas solve, [1] fused envelope / update /
+ * extrema passes, [2] segment seam fix-up and prefix, [3] end extension
+ * setup, noise, finalisation and queue overhead.  No
  * counterpart in the reference (its phase split used an in-kernel clock,
  * dbg_serial_split_v0.py:25-42). */
 int eemd_last_phases(eemd_ctx* ctx, double* ms4);
