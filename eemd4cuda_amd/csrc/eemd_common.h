@@ -49,18 +49,17 @@ struct TaskDesc {
 };
 
 // Per-workgroup scratch (global memory).  Offsets in bytes, 256-aligned.
-// Knot records are 32 bytes: FP64 {t, y, m, -}; FP32 {t, y, m, -, c1, c2, c3, -}
-// where c1..c3 are the Horner coefficients of the interval starting at the
-// knot (value = y + tau*(c1 + tau*(c2 + tau*c3)), tau = sample - t).
-constexpr int KREC_BYTES = 32;
+// Knot records are {t, y, M, -} in the working type: 16 bytes in FP32 mode,
+// 32 bytes in FP64 mode.
+constexpr int KREC_LEN = 4;
 constexpr int VSLOTS = 2 * MAX_VIRT;  // virtual-knot records after the segment regions
 
 struct ScratchLayout {
   int64_t h, r;       // working h and residual (T[npad])
   int64_t bits;       // extrema bitmaps uint8[2 sets][2 env][npad/8]
-  int64_t kr;         // knot records [2 sets][2 env][nseg*cap + VSLOTS] x 32 B
-  int64_t wcx;        // window Thomas forward sweep double2 {c', x'} [rows][cols]
-  int64_t ws;         // knot slot of each window row int32[rows][cols]
+  int64_t kr;         // knot records [2 sets][2 env][nseg*cap + VSLOTS] x 4 T
+  int64_t wcx;        // window Thomas forward sweep {c', x'} [rows][cols]
+                      // (double2 in FP64 mode, float2 in FP32 mode)
   int64_t total;
 };
 
@@ -73,9 +72,8 @@ __host__ __device__ inline ScratchLayout make_layout(int tbytes, int64_t npad, i
   L.h = o;    o = align256(o + npad * tbytes);
   L.r = o;    o = align256(o + npad * tbytes);
   L.bits = o; o = align256(o + 4 * (npad / 8));
-  L.kr = o;   o = align256(o + 4 * (nseg * cap + VSLOTS) * KREC_BYTES);
-  L.wcx = o;  o = align256(o + rows * cols * 16);
-  L.ws = o;   o = align256(o + rows * cols * 4);
+  L.kr = o;   o = align256(o + 4 * (nseg * cap + VSLOTS) * KREC_LEN * tbytes);
+  L.wcx = o;  o = align256(o + rows * cols * 2 * tbytes);
   L.total = o;
   return L;
 }

@@ -184,13 +184,12 @@ struct Mem {
   T* kr;            // knot records [2][2][NW*cap + VSLOTS], KREC_BYTES each
   int64_t rstride;  // records per (set, env)
   int64_t vbase;    // slot of virtual knot 0 (= NW*cap)
-  double2* wcx;     // window forward sweep {c', x'} [rows][cols]
-  int32_t* ws;      // slot code of each window row
+  void* wcx;        // window forward sweep {c', x'} [rows][cols] (T2)
   int64_t cols;
 };
 
 template <typename T>
-constexpr int rec_len() { return KREC_BYTES / (int)sizeof(T); }
+constexpr int rec_len() { return KREC_LEN; }
 
 template <typename T>
 __device__ __forceinline__ T* rec(const Mem<T>& M, int set, int env, int64_t slot) {
@@ -847,11 +846,6 @@ __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, in
   }
 }
 
-template <typename T>
-__device__ __forceinline__ int bcode_last_row(const Mem<T>& M, int w0, int w1, int col) {
-  return M.ws[(int64_t)(w1 - w0) * M.cols + col];
-}
-
 template <typename T, int NW>
 __device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int cur, int env,
                                         int code, double val) {
@@ -925,17 +919,80 @@ __device__ __forceinline__ void load_ty(const T* r, double& t, double& y) {
   }
 }
 
+// Reverse knot stream: slots of extended indices e, e-1, ... (the back
+// substitution walks the window from its last row to its first).
+struct KRev {
+  int e;         // extended index of the next knot to return
+  int w;         // segment of the next real knot
+  int left;      // real knots before `slot` in its segment run
+  int64_t slot;  // slot of the next real knot
+};
+
+template <typename T, int NW>
+__device__ __forceinline__ void kr_real_at(KRev& kr, const Shared<T, NW>& S, const Mem<T>& M,
+                                           int set, int env, int g) {
+  int w = NW - 1;
+  kr.slot = real_slot(S, M, set, env, g, w);
+  kr.w = w;
+  kr.left = g - S.pre[set][env][w];
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ void kr_seek(KRev& kr, const Shared<T, NW>& S, const Mem<T>& M, int set,
+                                        int env, int e) {
+  kr.e = e;
+  kr.left = 0;
+  const int g = e - S.nl[env];
+  if (g >= 0 && g < S.p[env]) kr_real_at(kr, S, M, set, env, g);
+}
+
+template <typename T, int NW>
+__device__ __forceinline__ int64_t kr_prev(KRev& kr, const Shared<T, NW>& S, const Mem<T>& M,
+                                           int set, int env, int nl, int p) {
+  const int g = kr.e - nl;
+  kr.e--;
+  if (g >= p) {
+    if (g == p && p > 0) kr_real_at(kr, S, M, set, env, p - 1);
+    return M.vbase + MAX_VIRT + (g - p);
+  }
+  if (g < 0) return M.vbase + (g + nl);
+  const int64_t s = kr.slot;
+  if (kr.left > 0) {
+    kr.slot--;
+    kr.left--;
+  } else if (g >= 1) {
+    int w = kr.w - 1;
+#pragma unroll 1
+    while (S.cnt[set][env][w] == 0) --w;
+    kr.w = w;
+    kr.left = S.cnt[set][env][w] - 1;
+    kr.slot = (int64_t)w * M.cap + S.beg[set][env][w] + kr.left;
+  }
+  return s;
+}
+
+// Division for the window recurrences: IEEE division in FP64 mode (the
+// oracle's rounding, emd_ref.py:58-61); in FP32 mode one FP64 reciprocal
+// per denominator.
+template <typename T>
+__device__ __forceinline__ double qdiv(double a, double b) {
+  if constexpr (std::is_same<T, double>::value) return a / b;
+  else return a * __builtin_amdgcn_rcp(b);
+}
+
 // Decay-seamed block Thomas over both envelopes of set `cur`: owned blocks of
 // B rows solved on the clipped window [o0-S, o1+S] with the couplings outside
 // the window dropped (emd_gpu_v5_dsbt.py:51-74, 223-237); one window per
 // thread.  B <= 0: one window per envelope (exact Thomas, emd_ref.py:50-65).
 // Rows are built on the fly from the knots (emd_ref.py:31-48).  The serial
-// recurrence is fed by a knot stream read PF rows ahead, and the back
-// substitution reads its forward-sweep rows PF ahead, so memory latency
-// overlaps the division chain.
+// recurrence is fed by a knot stream read PF rows ahead; the back
+// substitution reads its forward-sweep rows PF ahead and finds each row's
+// knot slot by walking the knots backwards.
 template <typename T, int NW>
 __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp) {
   constexpr int PF = 8;
+  using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+  T2* const wcx = reinterpret_cast<T2*>(M.wcx);
   int nwin[2], nrow[2], Be[2];
   for (int ev = 0; ev < 2; ++ev) {
     const int p = S.p[ev];
@@ -959,29 +1016,25 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     if (w0 < 0) w0 = 0;
     int w1 = o1 + Sp;
     if (w1 > n - 1) w1 = n - 1;
+    T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
 
     KStream ks;
     ks_seek(ks, S, M, cur, ev, w0);
-    double at, ay, bt, by;
-    load_ty(rec(M, cur, ev, ks_next(ks, S, M, cur, ev)), at, ay);
-    int bcode = (int)ks_next(ks, S, M, cur, ev);
-    load_ty(rec(M, cur, ev, bcode), bt, by);
-    double sl0 = (by - ay) / (bt - at);
+    const T2 ka = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    const T2 kb = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    double at = (double)ka.x, ay = (double)ka.y, bt = (double)kb.x, by = (double)kb.y;
+    double sl0 = qdiv<T>(by - ay, bt - at);
     double cprev = 0.0, xprev = 0.0;
 
     const int elast = w1 + 2;  // last knot the window's rows need
     int enext = w0 + 2;
     // prefetched knots stay in their storage type until used, so the loads
     // are not waited for at issue
-    using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
     T2 nk[PF];
-    int nc[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-      nc[u] = 0;
       if (enext <= elast) {
-        nc[u] = (int)ks_next(ks, S, M, cur, ev);
-        nk[u] = *reinterpret_cast<const T2*>(rec(M, cur, ev, nc[u]));
+        nk[u] = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
         ++enext;
       } else {
         nk[u].x = (T)0;
@@ -991,13 +1044,10 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
 #pragma unroll 1
     for (int q0 = w0; q0 <= w1; q0 += PF) {
       T2 fk[PF];
-      int fc[PF];
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
-        fc[u] = 0;
         if (enext <= elast) {
-          fc[u] = (int)ks_next(ks, S, M, cur, ev);
-          fk[u] = *reinterpret_cast<const T2*>(rec(M, cur, ev, fc[u]));
+          fk[u] = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
           ++enext;
         } else {
           fk[u].x = (T)0;
@@ -1014,14 +1064,22 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
           const double d = 2.0 * (H0 + H1);
           const double l = (q >= 1) ? H0 : 0.0;
           const double uu = (q <= n - 2) ? H1 : 0.0;
-          const double sl1 = (cy - by) / H1;
+          const double sl1 = qdiv<T>(cy - by, H1);
           const double r = 6.0 * (sl1 - sl0);
           const double den = d - l * cprev;
-          const double c = (q == w1) ? 0.0 : uu / den;
-          const double xv = (r - l * xprev) / den;
-          const int64_t row = (int64_t)(q - w0) * M.cols + col;
-          M.wcx[row] = make_double2(c, xv);
-          M.ws[row] = bcode;
+          double c, xv;
+          if constexpr (std::is_same<T, double>::value) {
+            c = (q == w1) ? 0.0 : uu / den;
+            xv = (r - l * xprev) / den;
+          } else {
+            const double inv = __builtin_amdgcn_rcp(den);
+            c = (q == w1) ? 0.0 : uu * inv;
+            xv = (r - l * xprev) * inv;
+          }
+          T2 cx;
+          cx.x = (T)c;
+          cx.y = (T)xv;
+          wcx[(int64_t)(q - w0) * M.cols + col] = cx;
           cprev = c;
           xprev = xv;
           sl0 = sl1;
@@ -1029,52 +1087,46 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
           ay = by;
           bt = ct;
           by = cy;
-          bcode = nc[u];
         }
       }
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        nk[u] = fk[u];
-        nc[u] = fc[u];
-      }
+      for (int u = 0; u < PF; ++u) nk[u] = fk[u];
     }
-    // back substitution, rows w1 .. w0, forward-sweep rows read PF ahead
+    // back substitution, rows w1 .. w0 (row q sets M of extended knot q+1)
+    const int nl = S.nl[ev], p = S.p[ev];
+    KRev kr;
+    kr_seek(kr, S, M, cur, ev, w1 + 1);
     double xn = xprev;
-    if (w1 >= o0 && w1 <= o1) store_m(S, M, cur, ev, bcode_last_row(M, w0, w1, col), xn);
-    double2 cx[PF];
-    int sc[PF];
+    {
+      const int64_t s1 = kr_prev(kr, S, M, cur, ev, nl, p);
+      if (w1 >= o0 && w1 <= o1) rb[s1 * rec_len<T>() + 2] = (T)xn;
+    }
+    T2 cx[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       const int q = w1 - 1 - u;
-      const int64_t row = (int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col;
-      cx[u] = M.wcx[row];
-      sc[u] = M.ws[row];
+      cx[u] = wcx[(int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col];
     }
 #pragma unroll 1
     for (int q0 = w1 - 1; q0 >= w0; q0 -= PF) {
-      double2 fx[PF];
-      int fs[PF];
+      T2 fx[PF];
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int q = q0 - PF - u;
-        const int64_t row = (int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col;
-        fx[u] = M.wcx[row];
-        fs[u] = M.ws[row];
+        fx[u] = wcx[(int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col];
       }
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         const int q = q0 - u;
         if (q >= w0) {
-          const double xi = cx[u].y - cx[u].x * xn;
-          if (q >= o0 && q <= o1) store_m(S, M, cur, ev, sc[u], xi);
+          const double xi = (double)cx[u].y - (double)cx[u].x * xn;
+          const int64_t sq = kr_prev(kr, S, M, cur, ev, nl, p);
+          if (q >= o0 && q <= o1) rb[sq * rec_len<T>() + 2] = (T)xi;
           xn = xi;
         }
       }
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        cx[u] = fx[u];
-        sc[u] = fs[u];
-      }
+      for (int u = 0; u < PF; ++u) cx[u] = fx[u];
     }
   }
 }
@@ -1101,8 +1153,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   M.kr = reinterpret_cast<T*>(scr + LY.kr);
   M.rstride = (int64_t)NW * A.cap + VSLOTS;
   M.vbase = (int64_t)NW * A.cap;
-  M.wcx = reinterpret_cast<double2*>(scr + LY.wcx);
-  M.ws = reinterpret_cast<int32_t*>(scr + LY.ws);
+  M.wcx = reinterpret_cast<void*>(scr + LY.wcx);
   M.cols = A.cols;
 
   const int trace = A.trace != nullptr;
