@@ -228,6 +228,8 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
     P.cols = 2;
   } else {
     P.rows = (int64_t)p->block + 2 * (int64_t)p->halo;
+    // FP32: room for the wave-parallel window solve (3 x 64 x 64 floats per wave)
+    if (!p->fp64) P.rows = std::max<int64_t>(P.rows, 3 * 64 * WAVE * 4 / (2 * 4 * WAVE));
     P.cols = P.threads;
   }
   P.stride = make_layout(tb, P.npad, nw, P.cap, P.rows, P.cols).total;

@@ -185,7 +185,7 @@ struct Mem {
   int64_t rstride;  // records per (set, env)
   int64_t vbase;    // slot of virtual knot 0 (= NW*cap)
   void* wcx;        // window forward sweep {c', x'} [rows][cols] (T2)
-  int64_t cols;
+  int64_t rows, cols;
 };
 
 template <typename T>
@@ -980,6 +980,137 @@ __device__ __forceinline__ double qdiv(double a, double b) {
   else return a * __builtin_amdgcn_rcp(b);
 }
 
+// One window [w0, w1] of rows (couplings outside the window dropped) solved
+// by a whole wave with the partition method: lane L eliminates its block of
+// sb rows downwards keeping the dependence on the row before the block
+// (x_i + c'_i x_{i+1} = d'_i + v_i alpha), sweeps back to express its first
+// and last unknowns in (alpha, beta), the wave solves the 2-per-lane
+// interface system serially with readlane broadcasts, and every lane
+// reconstructs its rows.  Exact up to rounding (FP32 mode only).
+template <typename T, int NW>
+__device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, int cur, int ev,
+                                            int n, int w0, int w1, int o0, int o1) {
+  const int ln = lane_id();
+  const int w = (int)(threadIdx.x / WAVE);
+  const int m = w1 - w0 + 1;
+  const int sb = (m + WAVE - 1) / WAVE;
+  const int nact = (m + sb - 1) / sb;
+  const int s = w0 + ln * sb;
+  const int e = (s + sb - 1 < w1) ? s + sb - 1 : w1;
+  const bool act = ln < nact;
+  T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
+  float* const scr = reinterpret_cast<float*>(M.wcx) + (int64_t)w * (3 * 64 * WAVE);
+  // scratch row j of this lane: scr[(j*3 + comp) * WAVE + ln], sb <= 64
+  double Ps = 0, Qs = 0, Rs = 0, Pe = 0, Qe = 0, Re = 0;
+  if (act) {
+    KStream ks;
+    ks_seek(ks, S, M, cur, ev, s);
+    float2 ka = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    float2 kb = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    double at = ka.x, ay = ka.y, bt = kb.x, by = kb.y;
+    double sl0 = (by - ay) * __builtin_amdgcn_rcp(bt - at);
+    double cp = 0.0, dp = 0.0, vp = 0.0;
+#pragma unroll 1
+    for (int q = s; q <= e; ++q) {
+      const float2 kc = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+      const double ct = kc.x, cy = kc.y;
+      const double H0 = bt - at, H1 = ct - bt;
+      const double d = 2.0 * (H0 + H1);
+      const double l = (q >= 1 && q > w0) ? H0 : 0.0;   // window edge: coupling dropped
+      const double u = (q <= n - 2 && q < w1) ? H1 : 0.0;
+      const double sl1 = (cy - by) * __builtin_amdgcn_rcp(H1);
+      const double r = 6.0 * (sl1 - sl0);
+      const double den = (q == s) ? d : d - l * cp;
+      const double inv = __builtin_amdgcn_rcp(den);
+      const double c = u * inv;
+      const double dd = ((q == s) ? r : r - l * dp) * inv;
+      const double vv = ((q == s) ? -l : -l * vp) * inv;
+      const int j = q - s;
+      scr[(j * 3 + 0) * WAVE + ln] = (float)c;
+      scr[(j * 3 + 1) * WAVE + ln] = (float)dd;
+      scr[(j * 3 + 2) * WAVE + ln] = (float)vv;
+      cp = c;
+      dp = dd;
+      vp = vv;
+      sl0 = sl1;
+      at = bt;
+      ay = by;
+      bt = ct;
+      by = cy;
+    }
+    // last row: x_e = d'_e + v_e alpha - c'_e beta
+    Pe = dp;
+    Qe = vp;
+    Re = -cp;
+    double P = Pe, Q = Qe, R = Re;
+#pragma unroll 1
+    for (int j = e - s - 1; j >= 0; --j) {
+      const double c = scr[(j * 3 + 0) * WAVE + ln];
+      const double dd = scr[(j * 3 + 1) * WAVE + ln];
+      const double vv = scr[(j * 3 + 2) * WAVE + ln];
+      P = dd - c * P;
+      Q = vv - c * Q;
+      R = -c * R;
+    }
+    Ps = P;
+    Qs = Q;
+    Rs = R;
+  }
+  // interface system: F_L = x_{s_L}, G_L = x_{e_L}; G_{-1} = F_{nact} = 0
+  //   F_L = Ps + Qs G_{L-1} + Rs F_{L+1},  G_L = Pe + Qe G_{L-1} + Re F_{L+1}
+  // forward: G_{L-1} = A + B F_L  ->  F_L = a_L + b_L F_{L+1},  G_L = A_L + B_L F_{L+1}
+  double A = 0.0, B = 0.0;
+  double my_a = 0.0, my_b = 0.0, my_A = 0.0, my_B = 0.0;
+#pragma unroll 1
+  for (int L = 0; L < nact; ++L) {
+    const double ps = __shfl(Ps, L), qs = __shfl(Qs, L), rs = __shfl(Rs, L);
+    const double pe = __shfl(Pe, L), qe = __shfl(Qe, L), re = __shfl(Re, L);
+    const double inv = __builtin_amdgcn_rcp(1.0 - qs * B);
+    const double a = (ps + qs * A) * inv;
+    const double b = rs * inv;
+    const double An = pe + qe * (A + B * a);
+    const double Bn = qe * B * b + re;
+    if (ln == L) {
+      my_a = a;
+      my_b = b;
+      my_A = An;
+      my_B = Bn;
+    }
+    A = An;
+    B = Bn;
+  }
+  double alpha = 0.0, beta = 0.0, Fn = 0.0;
+#pragma unroll 1
+  for (int L = nact - 1; L >= 0; --L) {
+    const double fL = __shfl(my_a + my_b * Fn, L);
+    const double gL = __shfl(my_A + my_B * Fn, L);
+    if (ln == L) beta = Fn;
+    if (ln == L + 1) alpha = gL;
+    Fn = fL;
+  }
+  if (act) {
+    // reconstruct rows e .. s (row q sets M of extended knot q+1)
+    const int nl = S.nl[ev], p = S.p[ev];
+    KRev kr;
+    kr_seek(kr, S, M, cur, ev, e + 1);
+    double x = Pe + Qe * alpha + Re * beta;
+    {
+      const int64_t sl = kr_prev(kr, S, M, cur, ev, nl, p);
+      if (e >= o0 && e <= o1) rb[sl * rec_len<T>() + 2] = (T)x;
+    }
+#pragma unroll 1
+    for (int j = e - s - 1; j >= 0; --j) {
+      const double c = scr[(j * 3 + 0) * WAVE + ln];
+      const double dd = scr[(j * 3 + 1) * WAVE + ln];
+      const double vv = scr[(j * 3 + 2) * WAVE + ln];
+      x = dd + vv * alpha - c * x;
+      const int64_t sl = kr_prev(kr, S, M, cur, ev, nl, p);
+      const int q = s + j;
+      if (q >= o0 && q <= o1) rb[sl * rec_len<T>() + 2] = (T)x;
+    }
+  }
+}
+
 // Decay-seamed block Thomas over both envelopes of set `cur`: owned blocks of
 // B rows solved on the clipped window [o0-S, o1+S] with the couplings outside
 // the window dropped (emd_gpu_v5_dsbt.py:51-74, 223-237); one window per
@@ -1003,6 +1134,33 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
   }
   const int total = nwin[0] + nwin[1];
   const int col = (int)threadIdx.x;
+  if constexpr (std::is_same<T, float>::value) {
+    // few long windows: a wave per window (partition method)
+    // needs blocks longer than a wave, windows of at most 64 rows per lane
+    // and the per-wave scratch (3 x 64 x 64 floats) inside the window area
+    const bool fits = M.cols >= (int64_t)blockDim.x &&
+                      M.rows * M.cols * 2 >= (int64_t)NW * 3 * 64 * WAVE &&
+                      Be[0] + 2 * Sp <= 64 * WAVE && Be[1] + 2 * Sp <= 64 * WAVE;
+    if (fits && total <= 8 * NW && Be[0] + Be[1] > 2 * WAVE) {
+      const int wv = (int)(threadIdx.x / WAVE);
+#pragma unroll 1
+      for (int wi = wv; wi < total; wi += NW) {
+        const int ev = wi < nwin[0] ? 0 : 1;
+        const int b = ev == 0 ? wi : wi - nwin[0];
+        const int n = nrow[ev];
+        const int o0 = b * Be[ev];
+        int o1 = o0 + Be[ev];
+        if (o1 > n) o1 = n;
+        o1 -= 1;
+        int w0 = o0 - Sp;
+        if (w0 < 0) w0 = 0;
+        int w1 = o1 + Sp;
+        if (w1 > n - 1) w1 = n - 1;
+        window_wave(S, M, cur, ev, n, w0, w1, o0, o1);
+      }
+      return;
+    }
+  }
 #pragma unroll 1
   for (int wi = (int)threadIdx.x; wi < total; wi += (int)blockDim.x) {
     const int ev = wi < nwin[0] ? 0 : 1;
@@ -1154,6 +1312,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   M.rstride = (int64_t)NW * A.cap + VSLOTS;
   M.vbase = (int64_t)NW * A.cap;
   M.wcx = reinterpret_cast<void*>(scr + LY.wcx);
+  M.rows = A.rows;
   M.cols = A.cols;
 
   const int trace = A.trace != nullptr;
