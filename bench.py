@@ -58,7 +58,7 @@ def workload(name: str):
 class DeviceRun:
     """Device-resident inputs/outputs for repeated eemd_run_batch_dev calls."""
 
-    def __init__(self, wl, fp: str, halo: int, threads: int = 0, mpt: int = 0):
+    def __init__(self, wl, fp: str, halo: int, threads: int = 0, mpt: int = 0, wave_windows: int = 0):
         import torch
         from eemd4cuda_amd import _lib
         self.torch = torch
@@ -87,6 +87,7 @@ class DeviceRun:
         p.halo = halo
         p.threads = threads
         p.members_per_task = mpt
+        p.wave_windows = wave_windows
         p.seed = 1
         self.p = p
         torch.cuda.synchronize()
@@ -114,9 +115,9 @@ class DeviceRun:
         return out
 
 
-def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0):
+def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0):
     import torch
-    r = DeviceRun(wl, fp, halo, threads=threads, mpt=mpt)
+    r = DeviceRun(wl, fp, halo, threads=threads, mpt=mpt, wave_windows=wave_windows)
     for _ in range(warmup):
         r.step()
     torch.cuda.synchronize()
@@ -191,6 +192,7 @@ def main():
     ap.add_argument("--mpt", type=int, default=0, help="members per task (0: automatic)")
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
+    ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the EEMD engine runs on a single GPU here "
@@ -204,7 +206,8 @@ def main():
         wl["E"] = a.E
     if a.max_imf:
         wl["max_imf"] = a.max_imf
-    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads, mpt=a.mpt)
+    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads, mpt=a.mpt,
+                         wave_windows=a.wave_windows)
     out = {
         "metric": "ensemble sample-sifts/s",
         "value": f32["value"],

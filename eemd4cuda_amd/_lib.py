@@ -23,7 +23,8 @@ class Params(C.Structure):
     _fields_ = [("fp64", C.c_int32), ("end_mode", C.c_int32), ("max_imf", C.c_int32),
                 ("max_sifts", C.c_int32), ("min_extrema", C.c_int32), ("block", C.c_int32),
                 ("halo", C.c_int32), ("threads", C.c_int32), ("members_per_task", C.c_int32),
-                ("flags", C.c_int32), ("tau", C.c_double), ("seed", C.c_uint64)]
+                ("flags", C.c_int32), ("tau", C.c_double), ("seed", C.c_uint64),
+                ("wave_windows", C.c_int32), ("reserved", C.c_int32)]
 
 
 class EEMDBackendError(RuntimeError):

@@ -305,6 +305,8 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.mlp = nullptr;
   A.nslots = P.grid;
   A.phase = nullptr;
+  A.wave_windows = p->wave_windows == 0 ? 8 * (P.threads / WAVE)
+                                        : (p->wave_windows < 0 ? -1 : p->wave_windows);
   if (p->flags & EEMD_FLAG_PHASES) {
     HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
     A.phase = (uint64_t*)c->phase.p;

@@ -116,6 +116,7 @@ struct SiftArgs {
   int64_t rows, cols;       // window scratch shape
   const void* mlp;          // learned-boundary model (optional)
   uint64_t* phase;          // optional per-workgroup phase clocks [nslots][4]
+  int32_t wave_windows;     // FP32: wave-parallel window solve when windows <= this
   int32_t nslots;           // number of workgroups (scratch slots)
   int32_t pad;
 };

@@ -1120,7 +1120,8 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
 // substitution reads its forward-sweep rows PF ahead and finds each row's
 // knot slot by walking the knots backwards.
 template <typename T, int NW>
-__device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp) {
+__device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp,
+                                      int wave_windows) {
   constexpr int PF = 8;
   using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
   T2* const wcx = reinterpret_cast<T2*>(M.wcx);
@@ -1141,7 +1142,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     const bool fits = M.cols >= (int64_t)blockDim.x &&
                       M.rows * M.cols * 2 >= (int64_t)NW * 3 * 64 * WAVE &&
                       Be[0] + 2 * Sp <= 64 * WAVE && Be[1] + 2 * Sp <= 64 * WAVE;
-    if (fits && total <= 8 * NW && Be[0] + Be[1] > 2 * WAVE) {
+    if (fits && total <= wave_windows && Be[0] + Be[1] > 2 * WAVE) {
       const int wv = (int)(threadIdx.x / WAVE);
 #pragma unroll 1
       for (int wi = wv; wi < total; wi += NW) {
@@ -1391,7 +1392,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           }
           __syncthreads();
           PHASE(3);
-          solve(S, M, cur, A.B, A.S);
+          solve(S, M, cur, A.B, A.S, A.wave_windows);
           __syncthreads();
           PHASE(0);
           const bool last = (s == A.max_sifts - 1);

@@ -70,6 +70,9 @@ typedef struct eemd_params {
   int32_t flags;            /* EEMD_FLAG_* */
   double tau;               /* SD stopping tolerance (emd_ref.py:140-146); <= 0: fixed sifts */
   uint64_t seed;            /* Philox key when no explicit noise is given */
+  int32_t wave_windows;     /* FP32: solve windows a wave at a time when a workgroup has at most
+                               this many (0: automatic, < 0: never) */
+  int32_t reserved;
 } eemd_params;
 
 /* Defaults: FP64, mirror, max_imf 10, max_sifts 10, min_extrema 4, B 1024, S 30,

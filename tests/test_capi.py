@@ -45,7 +45,7 @@ def test_params_default_without_gpu():
 
 def test_params_struct_layout_matches_header():
     from eemd4cuda_amd import _lib
-    assert ctypes.sizeof(_lib.Params) == 56
+    assert ctypes.sizeof(_lib.Params) == 64
 
 
 def test_decompose_input_validation():
