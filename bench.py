@@ -176,6 +176,19 @@ def cpu_baseline_c(wl, budget_cores: int):
                 sample="%d members, C oracle -O2 OpenMP, exact Thomas; wall %.1f s" % (E, wall))
 
 
+def copy_probe(mib: int):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,12 +206,18 @@ def main():
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: the EEMD engine runs on a single GPU here "
               "(--gpus 1).", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     wl = workload(a.workload)
     if a.block:
         wl["B"] = a.block

@@ -114,10 +114,12 @@ struct Plan {
   int64_t stride = 0;
 };
 
-int auto_threads(int64_t nmax) {
+int auto_threads(int64_t nmax, bool fp64) {
   // 1024-thread workgroups cap a lane at 128 VGPRs, below what the fused
-  // sift pass needs; 512 keeps every value in registers
-  if (nmax <= 8192) return 256;
+  // sift pass needs; 512 keeps every FP32 value in registers.  The FP64 pass
+  // holds twice the registers per sample and runs fastest at 256 (measured
+  // on the 2^20-sample, 256-member workload: 425 ms at 256, 525 ms at 512).
+  if (nmax <= 8192 || fp64) return 256;
   return 512;
 }
 
@@ -155,7 +157,7 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   const int tb = p->fp64 ? 8 : 4;
   P.nmax = 0;
   for (int s = 0; s < nsig; ++s) P.nmax = std::max<int64_t>(P.nmax, offsets[s + 1] - offsets[s]);
-  P.threads = p->threads ? p->threads : auto_threads(P.nmax);
+  P.threads = p->threads ? p->threads : auto_threads(P.nmax, p->fp64 != 0);
   const int nw = P.threads / WAVE;
 
   int bpc = 1;

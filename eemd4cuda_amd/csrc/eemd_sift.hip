@@ -17,11 +17,14 @@
 //   bits        1 bit per sample per envelope: bit b <=> sample b-1 is an
 //               extremum (the shift lets a chunk decide its bits from values
 //               it has already produced); two sets (current / next sift)
-//   kt, ky      compacted knots per (set, envelope), stored per segment region
-//               (the wave that owns a signal segment writes its knots to its
-//               own region, so no barrier is needed inside a pass)
-//   km          spline second derivatives of the current knots
-//   wc, wx, ws  forward sweep of the window Thomas solves
+//   kr          knot records {t, y, M, -} of 4 T per knot, per (set, envelope),
+//               stored per segment region (the wave that owns a signal segment
+//               writes its knots to its own region, so no barrier is needed
+//               inside a pass); VSLOTS records after the regions hold the
+//               virtual end knots.  M (spline second derivative) is written
+//               into the record by the solve.
+//   wcx         forward sweep {c', x'} of the window Thomas solves,
+//               [rows][cols] pairs, column = the lane that owns the window
 //
 // FP64 mode follows the oracle's arithmetic exactly (no contraction: the file
 // is compiled with -ffp-contract=off; IEEE divisions; the cube rounded once).
