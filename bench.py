@@ -92,14 +92,33 @@ class DeviceRun:
         self.p = p
         torch.cuda.synchronize()
 
-    def step(self, phases=False):
+    def step(self, phases=False, graph=False):
         from eemd4cuda_amd import _lib
-        self.p.flags = _lib.FLAG_PHASES if phases else 0
+        self.p.flags = (_lib.FLAG_PHASES if phases else 0) | (_lib.FLAG_GRAPH if graph else 0)
         self.eng.run_batch_dev(self.x.data_ptr(), self.offs, self.E, self.sig, self.p,
                                self.avg.data_ptr(), self.resid.data_ptr(),
                                self.n_modes.data_ptr(), self.nmode.data_ptr(),
                                self.nsift.data_ptr())
         return self.eng.last_timing()
+
+    def step_per_signal(self):
+        """The plain comparator: one engine call per signal, same arithmetic."""
+        from eemd4cuda_amd import _lib
+        self.p.flags = 0
+        for s in range(len(self.lens)):
+            o, n = int(self.offs[s]), int(self.lens[s])
+            self.eng.run_batch_dev(self.x.data_ptr() + 8 * o, np.array([0, n]), self.E,
+                                   self.sig[s:s + 1], self.p,
+                                   self.avg.data_ptr() + 8 * self.mi * o,
+                                   self.resid.data_ptr() + 8 * o,
+                                   self.n_modes.data_ptr() + 4 * s,
+                                   self.nmode.data_ptr() + 4 * s * self.E,
+                                   self.nsift.data_ptr() + 4 * s * self.E)
+
+    def autotune(self, reps=1):
+        best, ms = self.eng.autotune(self.x.data_ptr(), self.offs, self.E, self.sig, self.p,
+                                     reps=reps)
+        return best, ms
 
     def sample_sifts(self) -> int:
         ns = self.nsift.cpu().numpy().reshape(len(self.lens), self.E).astype(np.int64)
@@ -137,6 +156,45 @@ def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0):
                 threads=tm["threads"], grid=tm["grid"], ntasks=tm["ntasks"],
                 roofline=dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                               frac=achieved / HBM_PEAK_GBS, traffic=None)), r
+
+
+def _timed(r, steps, fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def comparators(wl, fp, steps, halo):
+    """Plain per-signal launches vs the batched persistent launch, then with
+    hipGraph replay and with the autotuned configuration (same arithmetic)."""
+    r = DeviceRun(wl, fp, halo)
+    r.step()
+    ss = r.sample_sifts()
+    out = {}
+    t_plain = _timed(r, 1, r.step_per_signal)
+    ss_plain = r.sample_sifts()
+    out["plain_per_signal"] = dict(value=ss_plain / t_plain, ms_per_step=t_plain * 1e3,
+                                   launches=len(r.lens))
+    t_b = _timed(r, steps, r.step)
+    out["batched"] = dict(value=ss / t_b, ms_per_step=t_b * 1e3)
+    r.step(graph=True)
+    t_g = _timed(r, steps, lambda: r.step(graph=True))
+    out["batched_graph"] = dict(value=ss / t_g, ms_per_step=t_g * 1e3)
+    t0 = time.perf_counter()
+    best, _ = r.autotune(reps=1)
+    tune_s = time.perf_counter() - t0
+    r.step(graph=True)   # automatic parameters now resolve to the tuned choice
+    t_t = _timed(r, steps, lambda: r.step(graph=True))
+    out["batched_graph_autotuned"] = dict(
+        value=ss / t_t, ms_per_step=t_t * 1e3, tune_s=tune_s,
+        choice=dict(threads=best.threads, members_per_task=best.members_per_task,
+                    wave_windows=best.wave_windows))
+    out["batched_vs_plain"] = out["batched_graph_autotuned"]["value"] / out["plain_per_signal"]["value"]
+    return out
 
 
 def cpu_baseline(wl, budget_cores: int):
@@ -206,6 +264,9 @@ def main():
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
+    ap.add_argument("--compare", action="store_true",
+                    help="also time plain per-signal launches, graph replay and the autotuned "
+                         "configuration (default for the uneven batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -264,6 +325,8 @@ def main():
                          for k in range(m)])
         out["fp32_imf_rel_l2_vs_fp64"] = errs[0] if len(errs) == 1 else \
             [float(np.max([e[k] for e in errs if len(e) > k])) for k in range(max(map(len, errs)))]
+    if a.compare or a.workload == "uneven":
+        out["comparators_f32"] = comparators(wl, "f32", max(1, a.steps // 2), a.halo32)
     prof = ROOT / "profiles" / ("traffic_%s.json" % a.workload)
     if prof.exists():
         tr = json.loads(prof.read_text())

@@ -18,6 +18,7 @@ END_MODES = {"natural": 0, "mirror": 1, "learned": 2}
 FLAG_MEMBER_IMFS = 1
 FLAG_TRACE = 2
 FLAG_PHASES = 4
+FLAG_GRAPH = 8
 
 class Params(C.Structure):
     _fields_ = [("fp64", C.c_int32), ("end_mode", C.c_int32), ("max_imf", C.c_int32),
@@ -41,6 +42,14 @@ def load(path: Path | None = None):
     if _lib is not None:
         return _lib
     p = Path(path) if path else lib_path()
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 (same
+    # soname as ROCm's); loading the engine first would bind ROCm's copy and
+    # a later torch import would start a second runtime that sees no GPU.
+    # With torch imported first the engine binds to torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not p.exists():
         raise RuntimeError(
             f"{p} not found: build it with `python -m eemd4cuda_amd.build` (hipcc, gfx950)")
@@ -56,11 +65,15 @@ def load(path: Path | None = None):
     L.eemd_run_batch.argtypes = run_args
     L.eemd_run_batch_dev.argtypes = run_args + [vp]
     L.eemd_noise.argtypes = [vp, u64, i32, i32, i32, i64, vp]
+    L.eemd_autotune.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.POINTER(Params), i32,
+                                C.POINTER(Params), C.POINTER(dbl)]
     L.eemd_last_timing.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(i32),
                                    C.POINTER(i32), C.POINTER(i32)]
     L.eemd_last_phases.argtypes = [vp, C.POINTER(dbl)]
+    L.eemd_graph_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     for name in ("eemd_ctx_create", "eemd_ctx_destroy", "eemd_run_batch", "eemd_run_batch_dev",
-                 "eemd_noise", "eemd_last_timing", "eemd_last_phases"):
+                 "eemd_noise", "eemd_last_timing", "eemd_last_phases", "eemd_autotune",
+                 "eemd_graph_stats"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -109,7 +122,7 @@ class Engine:
 
     def run_batch(self, x: np.ndarray, offsets: np.ndarray, E: int, sigmas: np.ndarray,
                   prm: Params, noise: np.ndarray | None = None, member_imfs: bool = False,
-                  trace: bool = False, per_mode: bool = True):
+                  trace: bool = False, per_mode: bool = True, graph: bool = False):
         """Host-buffer EEMD of the concatenated signals (eemd_run_batch)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
@@ -132,7 +145,7 @@ class Engine:
         nmode = np.zeros(nsig * E, np.int32)
         nsift = np.zeros(nsig * E, np.int32)
         pm = np.zeros(nsig * E * mi, np.int32) if per_mode else None
-        flags = 0
+        flags = FLAG_GRAPH if graph else 0
         mim = None
         if member_imfs:
             flags |= FLAG_MEMBER_IMFS
@@ -167,7 +180,7 @@ class Engine:
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         sigmas = np.ascontiguousarray(sigmas, dtype=np.float64)
         nsig = len(offsets) - 1
-        prm.flags = prm.flags & FLAG_PHASES
+        prm.flags = prm.flags & (FLAG_PHASES | FLAG_GRAPH)
         vp = C.c_void_p
         with _lock:
             rc = self.lib.eemd_run_batch_dev(
@@ -178,6 +191,21 @@ class Engine:
                 vp(stream) if stream else None)
         self._check(rc, "eemd_run_batch_dev")
 
+    def autotune(self, x_ptr: int, offsets: np.ndarray, E: int, sigmas: np.ndarray, prm: Params,
+                 noise_ptr: int | None = None, reps: int = 2):
+        """eemd_autotune on a device-resident batch: (best Params, kernel ms)."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        sigmas = np.ascontiguousarray(sigmas, dtype=np.float64)
+        best = Params()
+        ms = C.c_double()
+        with _lock:
+            rc = self.lib.eemd_autotune(self.h, C.c_void_p(x_ptr), _ptr(offsets), len(offsets) - 1,
+                                        int(E), _ptr(sigmas),
+                                        C.c_void_p(noise_ptr) if noise_ptr else None,
+                                        C.byref(prm), int(reps), C.byref(best), C.byref(ms))
+        self._check(rc, "eemd_autotune")
+        return best, ms.value
+
     def noise(self, seed: int, signal: int, member0: int, E: int, n: int) -> np.ndarray:
         """The in-kernel Philox normals, materialised: [E, n]."""
         out = np.zeros((E, n))
@@ -185,6 +213,12 @@ class Engine:
                                  int(member0), int(E), int(n), _ptr(out))
         self._check(rc, "eemd_noise")
         return out
+
+    def graph_stats(self) -> tuple:
+        """(graph captures, graph launches) of this context."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(self.lib.eemd_graph_stats(self.h, C.byref(a), C.byref(b)), "eemd_graph_stats")
+        return a.value, b.value
 
     def last_phases(self) -> list:
         out = (C.c_double * 4)()

@@ -64,6 +64,37 @@ struct DevBuf {
 
 }  // namespace
 
+// Signal / task descriptors, queue order and signal starts of one launch.
+struct DescBufs {
+  DevBuf sigd, taskd, order, sig_start;
+  void release() {
+    sigd.release();
+    taskd.release();
+    order.release();
+    sig_start.release();
+  }
+};
+
+// One instantiated launch sequence (clears -> sift -> mean) and the exact
+// call it was captured for.  It owns its descriptors, so eager calls in
+// between do not disturb it.
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<uint8_t> key;
+  DescBufs desc;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
+// Launch configuration chosen by eemd_autotune for one batch shape.
+struct TuneEntry {
+  std::vector<int64_t> key;
+  int threads, mpt, wave_windows;
+};
+
 struct eemd_ctx {
   int device = 0;
   int ncu = 0;
@@ -71,16 +102,34 @@ struct eemd_ctx {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   std::string err;
   // device buffers owned by the context
-  DevBuf scratch, acc, acc_cnt, sigd, taskd, order, sig_start, queue, phase;
+  DevBuf scratch, acc, acc_cnt, queue, phase;
+  DescBufs desc;
   // staging for the host-pointer entry point
   DevBuf hx, hnoise, havg, hres, hnm, hnmode, hnsift, hpm, hmi, htr;
+  // outputs of the autotuner's timed runs
+  DevBuf tx, tavg, tres, tnm, tnmode, tnsift, tpm;
+  GraphCache graph;
+  std::vector<TuneEntry> tuned;
   double last_sift_ms = 0.0, last_mean_ms = 0.0;
   int last_threads = 0, last_grid = 0, last_ntask = 0;
   bool timing_pending = false;
+  bool last_graph = false;
+  int64_t graph_captures = 0, graph_launches = 0;
   int last_phase_grid = 0;
 };
 
 namespace {
+
+// Kernel times of the last call from its events (a graph replay is timed
+// as a whole and reported as the sift time).
+hipError_t read_timing(eemd_ctx* c) {
+  float a = 0.f, b = 0.f;
+  hipError_t e = hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+  if (e == hipSuccess && !c->last_graph) e = hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+  c->last_sift_ms = a;
+  c->last_mean_ms = b;
+  return e;
+}
 
 int fail(eemd_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -109,6 +158,9 @@ struct Plan {
   int64_t acc_doubles = 0;
   int64_t units = 0;
   int threads = 0;
+  int mpt = 0;
+  int wave_windows = 0;
+  bool tuned = false;
   int grid = 0;
   int64_t npad = 0, cap = 0, rows = 0, cols = 0;
   int64_t stride = 0;
@@ -152,13 +204,35 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
   return EEMD_OK;
 }
 
+// Shape of a batch as far as the launch configuration is concerned.
+std::vector<int64_t> tune_key(const int64_t* offsets, int32_t nsig, int32_t E, const eemd_params* p) {
+  int64_t nmax = 0;
+  for (int s = 0; s < nsig; ++s) nmax = std::max<int64_t>(nmax, offsets[s + 1] - offsets[s]);
+  return {p->fp64, p->end_mode, p->max_imf, p->max_sifts, p->block, p->halo, nsig, E,
+          offsets[nsig], nmax};
+}
+
 int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const double* sigmas,
               const double* noise, const eemd_params* p, bool want_mimf, Plan& P) {
   const int tb = p->fp64 ? 8 : 4;
   P.nmax = 0;
   for (int s = 0; s < nsig; ++s) P.nmax = std::max<int64_t>(P.nmax, offsets[s + 1] - offsets[s]);
-  P.threads = p->threads ? p->threads : auto_threads(P.nmax, p->fp64 != 0);
+  int want_threads = p->threads, want_mpt = p->members_per_task, want_ww = p->wave_windows;
+  P.tuned = false;
+  if (want_threads == 0 && want_mpt == 0 && want_ww == 0) {
+    const std::vector<int64_t> key = tune_key(offsets, nsig, E, p);
+    for (const TuneEntry& t : c->tuned)
+      if (t.key == key) {
+        want_threads = t.threads;
+        want_mpt = t.mpt;
+        want_ww = t.wave_windows;
+        P.tuned = true;
+        break;
+      }
+  }
+  P.threads = want_threads ? want_threads : auto_threads(P.nmax, p->fp64 != 0);
   const int nw = P.threads / WAVE;
+  P.wave_windows = want_ww == 0 ? 8 * nw : (want_ww < 0 ? -1 : want_ww);
 
   int bpc = 1;
   hipError_t oe = sift_occupancy(p->fp64, P.threads, &bpc);
@@ -166,15 +240,20 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   if (bpc < 1) bpc = 1;
   const int64_t max_slots = (int64_t)bpc * c->ncu;
 
-  // members per task: enough tasks to fill every slot twice, but accumulators
-  // stay proportional to the number of tasks
-  int mpt = p->members_per_task;
-  if (mpt <= 0) {
-    const int64_t units = (int64_t)nsig * E;
-    const int64_t target = std::max<int64_t>(2 * max_slots, nsig);
-    mpt = (int)std::max<int64_t>(1, units / target);
-    mpt = std::min(mpt, E);
-  }
+  // Members per task.  An explicit value applies to every signal.  The
+  // automatic rule is per signal: a task of signal s holds at most
+  // W / (2 * slots * n_s) members (W = total sample-members), so no task is
+  // more than half of one slot's fair share and the longest-first queue can
+  // balance the tail, while short signals keep few tasks and few
+  // accumulators (one max_imf x n_s FP64 block per task).
+  int64_t work = 0;
+  for (int s = 0; s < nsig; ++s) work += (offsets[s + 1] - offsets[s]) * (int64_t)E;
+  auto mpt_of = [&](int64_t n) -> int {
+    if (want_mpt > 0) return std::min(want_mpt, E);
+    const int64_t m = work / (2 * max_slots * std::max<int64_t>(n, 1));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(m, E));
+  };
+  P.mpt = want_mpt > 0 ? std::min(want_mpt, E) : 0;
 
   P.sig.resize(nsig);
   P.start.resize(nsig);
@@ -192,6 +271,7 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
     d.unit0 = s * E;
     d.task0 = (int32_t)P.task.size();
     d.nmember = E;
+    const int mpt = mpt_of(n);
     for (int m0 = 0; m0 < E; m0 += mpt) {
       TaskDesc t{};
       t.signal = s;
@@ -238,6 +318,70 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   return EEMD_OK;
 }
 
+hipError_t upload_desc(const Plan& P, DescBufs& D, hipStream_t st) {
+  const size_t nt = P.task.size();
+  hipError_t e;
+  if ((e = D.sigd.ensure(sizeof(SignalDesc) * P.sig.size())) != hipSuccess) return e;
+  if ((e = D.taskd.ensure(sizeof(TaskDesc) * nt)) != hipSuccess) return e;
+  if ((e = D.order.ensure(sizeof(int32_t) * nt)) != hipSuccess) return e;
+  if ((e = D.sig_start.ensure(sizeof(int64_t) * P.start.size())) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(D.sigd.p, P.sig.data(), sizeof(SignalDesc) * P.sig.size(),
+                          hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(D.taskd.p, P.task.data(), sizeof(TaskDesc) * nt,
+                          hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(D.order.p, P.order.data(), sizeof(int32_t) * nt,
+                          hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+  return hipMemcpyAsync(D.sig_start.p, P.start.data(), sizeof(int64_t) * P.start.size(),
+                        hipMemcpyHostToDevice, st);
+}
+
+// The device work of one call: clears (fill kernels, never memsets, so the
+// sequence can be captured), the persistent sift kernel and the mean kernel.
+// With `events`, ev[0..2] bracket the sift and mean kernels.
+hipError_t enqueue_work(eemd_ctx* c, const Plan& P, SiftArgs A, const eemd_params* p,
+                        const DescBufs& D, const double* x, double* avg, double* resid,
+                        int32_t* n_modes, hipStream_t st, bool events) {
+  hipError_t e;
+  A.sig = (const SignalDesc*)D.sigd.p;
+  A.task = (const TaskDesc*)D.taskd.p;
+  A.order = (const int32_t*)D.order.p;
+  if ((e = launch_fill_i32((int32_t*)c->queue.p, 1, 0, st)) != hipSuccess) return e;
+  if ((e = launch_fill_f64(avg, (int64_t)p->max_imf * P.total_n, 0.0, st)) != hipSuccess) return e;
+  const int tb = p->fp64 ? 8 : 4;
+  if (A.member_imfs) {
+    int64_t bytes = 0;
+    for (const auto& s : P.sig) bytes += (int64_t)s.nmember * p->max_imf * s.n * tb;
+    if ((e = launch_fill_bytes(A.member_imfs, bytes, st)) != hipSuccess) return e;
+  }
+  if (A.trace) {
+    const int64_t words = P.units * p->max_imf * std::max(p->max_sifts, 1) * 4;
+    if ((e = launch_fill_bytes(A.trace, words * 8, st)) != hipSuccess) return e;
+  }
+  if (events && (e = hipEventRecord(c->ev[0], st)) != hipSuccess) return e;
+  if ((e = launch_sift(A, p->fp64, P.threads, P.grid, st)) != hipSuccess) return e;
+  if (events && (e = hipEventRecord(c->ev[1], st)) != hipSuccess) return e;
+  if ((e = launch_mean((const SignalDesc*)D.sigd.p, (int)P.sig.size(),
+                       (const int64_t*)D.sig_start.p, (const TaskDesc*)D.taskd.p,
+                       (const double*)c->acc.p, (const int32_t*)c->acc_cnt.p, p->max_imf, x, avg,
+                       resid, n_modes, P.total_n, st)) != hipSuccess)
+    return e;
+  if (events && (e = hipEventRecord(c->ev[2], st)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+template <typename V>
+void key_put(std::vector<uint8_t>& k, const V& v) {
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(&v);
+  k.insert(k.end(), b, b + sizeof(V));
+}
+
+template <typename V>
+void key_put_vec(std::vector<uint8_t>& k, const std::vector<V>& v) {
+  key_put(k, v.size());
+  const uint8_t* b = reinterpret_cast<const uint8_t*>(v.data());
+  k.insert(k.end(), b, b + sizeof(V) * v.size());
+}
+
 int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
              const eemd_params* p, double* avg, double* resid, int32_t* n_modes, int32_t* nmode,
              int32_t* nsift, int32_t* per_mode, void* mimf, uint64_t* trace, hipStream_t st,
@@ -246,40 +390,12 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   HIPCHK(c, c->scratch.ensure((size_t)P.stride * P.grid), "scratch alloc");
   HIPCHK(c, c->acc.ensure(sizeof(double) * (size_t)std::max<int64_t>(P.acc_doubles, 1)), "acc alloc");
   HIPCHK(c, c->acc_cnt.ensure(sizeof(int32_t) * nt * p->max_imf), "acc_cnt alloc");
-  HIPCHK(c, c->sigd.ensure(sizeof(SignalDesc) * P.sig.size()), "desc alloc");
-  HIPCHK(c, c->taskd.ensure(sizeof(TaskDesc) * nt), "desc alloc");
-  HIPCHK(c, c->order.ensure(sizeof(int32_t) * nt), "desc alloc");
-  HIPCHK(c, c->sig_start.ensure(sizeof(int64_t) * P.start.size()), "desc alloc");
   HIPCHK(c, c->queue.ensure(256), "queue alloc");
-  HIPCHK(c, hipMemcpyAsync(c->sigd.p, P.sig.data(), sizeof(SignalDesc) * P.sig.size(),
-                           hipMemcpyHostToDevice, st), "desc upload");
-  HIPCHK(c, hipMemcpyAsync(c->taskd.p, P.task.data(), sizeof(TaskDesc) * nt,
-                           hipMemcpyHostToDevice, st), "desc upload");
-  HIPCHK(c, hipMemcpyAsync(c->order.p, P.order.data(), sizeof(int32_t) * nt,
-                           hipMemcpyHostToDevice, st), "desc upload");
-  HIPCHK(c, hipMemcpyAsync(c->sig_start.p, P.start.data(), sizeof(int64_t) * P.start.size(),
-                           hipMemcpyHostToDevice, st), "desc upload");
-  HIPCHK(c, launch_fill_i32((int32_t*)c->queue.p, 1, 0, st), "queue reset");
-  HIPCHK(c, launch_fill_f64(avg, (int64_t)p->max_imf * P.total_n, 0.0, st), "mean clear");
-  const int tb = p->fp64 ? 8 : 4;
-  if (mimf) {
-    int64_t cnt = (int64_t)P.units * p->max_imf;  // per-unit blocks differ in n
-    int64_t bytes = 0;
-    for (const auto& s : P.sig) bytes += (int64_t)s.nmember * p->max_imf * s.n * tb;
-    (void)cnt;
-    HIPCHK(c, launch_fill_bytes(mimf, bytes, st), "member imf clear");
-  }
-  if (trace) {
-    const int64_t words = P.units * p->max_imf * std::max(p->max_sifts, 1) * 4;
-    HIPCHK(c, launch_fill_bytes(trace, words * 8, st), "trace clear");
-  }
 
-  SiftArgs A{};
+  SiftArgs A;
+  std::memset(&A, 0, sizeof(A));  // padding included: A is part of the graph key
   A.x = x;
   A.noise = noise;
-  A.sig = (const SignalDesc*)c->sigd.p;
-  A.task = (const TaskDesc*)c->taskd.p;
-  A.order = (const int32_t*)c->order.p;
   A.ntask = (int32_t)nt;
   A.nsig = (int32_t)P.sig.size();
   A.queue = (int32_t*)c->queue.p;
@@ -307,34 +423,71 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.mlp = nullptr;
   A.nslots = P.grid;
   A.phase = nullptr;
-  A.wave_windows = p->wave_windows == 0 ? 8 * (P.threads / WAVE)
-                                        : (p->wave_windows < 0 ? -1 : p->wave_windows);
+  A.wave_windows = P.wave_windows;
   if (p->flags & EEMD_FLAG_PHASES) {
     HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
     A.phase = (uint64_t*)c->phase.p;
   }
   c->last_phase_grid = (p->flags & EEMD_FLAG_PHASES) ? P.grid : 0;
 
-  HIPCHK(c, hipEventRecord(c->ev[0], st), "event");
-  HIPCHK(c, launch_sift(A, p->fp64, P.threads, P.grid, st), "sift kernel launch");
-  HIPCHK(c, hipEventRecord(c->ev[1], st), "event");
-  HIPCHK(c, launch_mean((const SignalDesc*)c->sigd.p, (int)P.sig.size(),
-                        (const int64_t*)c->sig_start.p, (const TaskDesc*)c->taskd.p,
-                        (const double*)c->acc.p, (const int32_t*)c->acc_cnt.p, p->max_imf, x, avg,
-                        resid, n_modes, P.total_n, st),
-         "mean kernel launch");
-  HIPCHK(c, hipEventRecord(c->ev[2], st), "event");
+  if (p->flags & EEMD_FLAG_GRAPH) {
+    // everything the captured sequence depends on: the plan, the parameters
+    // and every device address it was captured with
+    std::vector<uint8_t> key;
+    key_put_vec(key, P.sig);
+    key_put_vec(key, P.task);
+    key_put_vec(key, P.order);
+    key_put_vec(key, P.start);
+    eemd_params pk = *p;
+    key_put(key, pk);
+    key_put(key, P.threads);
+    key_put(key, P.grid);
+    key_put(key, P.stride);
+    key_put(key, P.wave_windows);
+    key_put(key, A);
+    GraphCache& G = c->graph;
+    if (!G.exec || G.key != key) {
+      G.reset();
+      hipStream_t cs = c->stream;
+      if (st != cs) HIPCHK(c, hipStreamSynchronize(st), "stream sync before capture");
+      HIPCHK(c, upload_desc(P, G.desc, cs), "descriptor upload");
+      HIPCHK(c, hipStreamSynchronize(cs), "descriptor upload");
+      HIPCHK(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+      hipError_t e = enqueue_work(c, P, A, p, G.desc, x, avg, resid, n_modes, cs, false);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(cs, &g);
+      if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail(c, e, "graph capture");
+      }
+      if (e2 != hipSuccess) return hip_fail(c, e2, "end capture");
+      e = hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) {
+        G.exec = nullptr;
+        return hip_fail(c, e, "graph instantiate");
+      }
+      G.key = key;
+      ++c->graph_captures;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], st), "event");
+    HIPCHK(c, hipGraphLaunch(G.exec, st), "graph launch");
+    HIPCHK(c, hipEventRecord(c->ev[1], st), "event");
+    HIPCHK(c, hipEventRecord(c->ev[2], st), "event");
+    c->last_graph = true;
+    ++c->graph_launches;
+  } else {
+    HIPCHK(c, upload_desc(P, c->desc, st), "descriptor upload");
+    HIPCHK(c, enqueue_work(c, P, A, p, c->desc, x, avg, resid, n_modes, st, true), "kernel launch");
+    c->last_graph = false;
+  }
   c->last_threads = P.threads;
   c->last_grid = P.grid;
   c->last_ntask = (int)nt;
   c->timing_pending = !sync_timing;
   if (sync_timing) {
     HIPCHK(c, hipStreamSynchronize(st), "kernel execution");
-    float a = 0.f, b = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "event timing");
-    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "event timing");
-    c->last_sift_ms = a;
-    c->last_mean_ms = b;
+    HIPCHK(c, read_timing(c), "event timing");
   }
   return EEMD_OK;
 }
@@ -393,9 +546,12 @@ int eemd_ctx_destroy(eemd_ctx* c) {
   if (!c) return EEMD_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->sigd, &c->taskd, &c->order,
-                    &c->sig_start, &c->queue, &c->phase, &c->hx, &c->hnoise, &c->havg, &c->hres, &c->hnm,
-                    &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr})
+  c->graph.reset();
+  c->graph.desc.release();
+  c->desc.release();
+  for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->queue, &c->phase, &c->hx, &c->hnoise,
+                    &c->havg, &c->hres, &c->hnm, &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr,
+                    &c->tx, &c->tavg, &c->tres, &c->tnm, &c->tnmode, &c->tnsift, &c->tpm})
     b->release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -492,6 +648,117 @@ int eemd_run_batch(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t
   return EEMD_OK;
 }
 
+int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t nsig, int32_t E,
+                  const double* sigmas, const double* noise, const eemd_params* prm, int32_t reps,
+                  eemd_params* best, double* best_ms) {
+  int rc = validate(c, offsets, nsig, E, sigmas, prm);
+  if (rc) return rc;
+  if (!x || !best) return fail(c, EEMD_ERR_BAD_ARG, "x and best are required");
+  if (reps < 1) reps = 1;
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  const int64_t ntot = offsets[nsig];
+  const int64_t units = (int64_t)nsig * E;
+  HIPCHK(c, c->tavg.ensure(sizeof(double) * prm->max_imf * ntot), "tune alloc");
+  HIPCHK(c, c->tres.ensure(sizeof(double) * ntot), "tune alloc");
+  HIPCHK(c, c->tnm.ensure(sizeof(int32_t) * nsig), "tune alloc");
+  HIPCHK(c, c->tnmode.ensure(sizeof(int32_t) * units), "tune alloc");
+  HIPCHK(c, c->tnsift.ensure(sizeof(int32_t) * units), "tune alloc");
+  HIPCHK(c, c->tpm.ensure(sizeof(int32_t) * units * prm->max_imf), "tune alloc");
+
+  // kernel time of one configuration (explicit threads / mpt / wave_windows)
+  auto timed = [&](int threads, int mpt, int ww, double& ms, int& slots) -> int {
+    eemd_params q = *prm;
+    q.threads = threads;
+    q.members_per_task = mpt;
+    q.wave_windows = ww;
+    q.flags = 0;
+    Plan P;
+    int r = make_plan(c, offsets, nsig, E, sigmas, noise, &q, false, P);
+    if (r) return r;
+    int bpc = 1;
+    HIPCHK(c, sift_occupancy(q.fp64, P.threads, &bpc), "occupancy query");
+    slots = std::max(1, bpc) * c->ncu;
+    ms = 1e300;
+    for (int k = 0; k < reps; ++k) {
+      r = run_plan(c, P, x, noise, &q, (double*)c->tavg.p, (double*)c->tres.p, (int32_t*)c->tnm.p,
+                   (int32_t*)c->tnmode.p, (int32_t*)c->tnsift.p, (int32_t*)c->tpm.p, nullptr,
+                   nullptr, c->stream, true);
+      if (r) return r;
+      ms = std::min(ms, c->last_sift_ms + c->last_mean_ms);
+    }
+    return EEMD_OK;
+  };
+  // accumulator bytes of a configuration (tasks x max_imf x n x 8)
+  auto acc_bytes = [&](int threads, int mpt) -> int64_t {
+    eemd_params q = *prm;
+    q.threads = threads;
+    q.members_per_task = mpt;
+    q.wave_windows = -1;
+    Plan P;
+    if (make_plan(c, offsets, nsig, E, sigmas, noise, &q, false, P)) return INT64_MAX;
+    return P.acc_doubles * 8;
+  };
+
+  // 1. workgroup size (automatic members per task, default wave threshold)
+  int bt = 0, bm = 0, bw = 0, slots = 1;
+  double bms = 1e300;
+  for (int t : {256, 512, 1024}) {
+    const int w = prm->fp64 ? -1 : 8 * (t / WAVE);
+    double ms;
+    rc = timed(t, 0, w, ms, slots);
+    if (rc) return rc;
+    if (ms < bms) { bms = ms; bt = t; bm = 0; bw = w; }
+  }
+  // 2. members per task: uniform choices, skipping those that leave slots
+  // idle or need far more accumulator memory than the automatic rule
+  {
+    int bpc = 1;
+    HIPCHK(c, sift_occupancy(prm->fp64, bt, &bpc), "occupancy query");
+    const int sl = std::max(1, bpc) * c->ncu;
+    const int64_t auto_acc = acc_bytes(bt, 0);
+    std::vector<int> cand;
+    for (int m : {1, 2, 4, 8, 16, E})
+      if (m <= E && std::find(cand.begin(), cand.end(), m) == cand.end()) cand.push_back(m);
+    for (int m : cand) {
+      const int64_t ntask = (int64_t)nsig * ((E + m - 1) / m);
+      if (ntask < sl) continue;
+      const int64_t ab = acc_bytes(bt, m);
+      if (ab > 2 * auto_acc && ab > (int64_t(4) << 30)) continue;
+      double ms;
+      rc = timed(bt, m, bw, ms, slots);
+      if (rc) return rc;
+      if (ms < bms) { bms = ms; bm = m; }
+    }
+  }
+  // 3. FP32 wave-solve threshold
+  if (!prm->fp64) {
+    const int nw = bt / WAVE;
+    for (int w : {-1, 2 * nw, 32 * nw}) {
+      if (w == bw) continue;
+      double ms;
+      rc = timed(bt, bm, w, ms, slots);
+      if (rc) return rc;
+      if (ms < bms) { bms = ms; bw = w; }
+    }
+  }
+  const std::vector<int64_t> key = tune_key(offsets, nsig, E, prm);
+  bool found = false;
+  for (TuneEntry& t : c->tuned)
+    if (t.key == key) {
+      t.threads = bt;
+      t.mpt = bm;
+      t.wave_windows = bw;
+      found = true;
+    }
+  if (!found) c->tuned.push_back(TuneEntry{key, bt, bm, bw});
+  *best = *prm;
+  best->threads = bt;
+  best->members_per_task = bm;
+  best->wave_windows = bw;
+  if (best_ms) *best_ms = bms;
+  return EEMD_OK;
+}
+
 int eemd_noise(eemd_ctx* c, uint64_t seed, int32_t signal, int32_t member0, int32_t E, int64_t n,
                double* out) {
   if (!c) return EEMD_ERR_BAD_ARG;
@@ -514,11 +781,7 @@ int eemd_last_timing(eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* thr
   if (c->timing_pending) {
     // the last call was enqueued on a caller stream: wait for its events
     HIPCHK(c, hipEventSynchronize(c->ev[2]), "event wait");
-    float a = 0.f, b = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]), "event timing");
-    HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]), "event timing");
-    c->last_sift_ms = a;
-    c->last_mean_ms = b;
+    HIPCHK(c, read_timing(c), "event timing");
     c->timing_pending = false;
   }
   if (sift_ms) *sift_ms = c->last_sift_ms;
@@ -526,6 +789,13 @@ int eemd_last_timing(eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* thr
   if (threads) *threads = c->last_threads;
   if (grid) *grid = c->last_grid;
   if (ntasks) *ntasks = c->last_ntask;
+  return EEMD_OK;
+}
+
+int eemd_graph_stats(eemd_ctx* c, int64_t* captures, int64_t* launches) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (launches) *launches = c->graph_launches;
   return EEMD_OK;
 }
 

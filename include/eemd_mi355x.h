@@ -54,6 +54,8 @@ extern "C" {
 #define EEMD_FLAG_MEMBER_IMFS 1 /* also return every member's IMFs */
 #define EEMD_FLAG_TRACE 2       /* per-sift extrema counts + index digests */
 #define EEMD_FLAG_PHASES 4      /* per-workgroup phase clocks (eemd_last_phases) */
+#define EEMD_FLAG_GRAPH 8       /* capture the launch sequence in a hipGraph and replay it on
+                                   later calls with the same plan, parameters and buffers */
 
 typedef struct eemd_ctx eemd_ctx;
 
@@ -121,6 +123,21 @@ int eemd_run_batch_dev(eemd_ctx* ctx, const double* x, const int64_t* offsets, i
                        int32_t* n_modes, int32_t* nmode, int32_t* nsift, int32_t* per_mode_sifts,
                        void* member_imfs, uint64_t* trace, void* stream);
 
+/* Times candidate launch configurations on the given device-resident batch
+ * (x and noise as for eemd_run_batch_dev; the timed runs write to context
+ * buffers) and returns the fastest: *best = *prm with threads,
+ * members_per_task and (FP32) wave_windows set, *best_ms its kernel time.
+ * The context remembers the choice for this batch shape (lengths, E, FP mode,
+ * IMF/sift counts, block, halo, end mode) and applies it to later calls whose
+ * threads, members_per_task and wave_windows are all 0.  Coordinate search:
+ * workgroup size, then members per task, then the FP32 wave-solve threshold;
+ * each candidate runs `reps` times and keeps its fastest run.  The reference
+ * picks its launch shape by hand (emd_gpu_v5_dsbt.py:343, K / nthreads / B
+ * arguments of gpu_eemd_full). */
+int eemd_autotune(eemd_ctx* ctx, const double* x, const int64_t* offsets, int32_t nsig, int32_t E,
+                  const double* sigmas, const double* noise, const eemd_params* prm, int32_t reps,
+                  eemd_params* best, double* best_ms);
+
 /* The standard normals member e of signal `signal` receives when no explicit
  * noise is given: out[e][i] for members member0 .. member0+E-1 (host memory). */
 int eemd_noise(eemd_ctx* ctx, uint64_t seed, int32_t signal, int32_t member0, int32_t E,
@@ -128,9 +145,15 @@ int eemd_noise(eemd_ctx* ctx, uint64_t seed, int32_t signal, int32_t member0, in
 
 /* Device time (ms, HIP events on the launch stream) of the last call's
  * persistent sift kernel and mean kernel, and its launch configuration.
- * After an asynchronous eemd_run_batch_dev call this waits for its events. */
+ * After an asynchronous eemd_run_batch_dev call this waits for its events.
+ * A call replayed from a graph (EEMD_FLAG_GRAPH) reports the whole graph in
+ * sift_ms and 0 in mean_ms. */
 int eemd_last_timing(eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* threads,
                      int32_t* grid, int32_t* ntasks);
+
+/* Graph captures and graph launches (EEMD_FLAG_GRAPH calls) since the context
+ * was created: a replayed call adds a launch without a capture. */
+int eemd_graph_stats(eemd_ctx* ctx, int64_t* captures, int64_t* launches);
 
 /* Mean per-workgroup time (ms) of the last call's phases when it ran with
  * EEMD_FLAG_PHASES: [0] block-Thomas solve, [1] fused envelope / update /

@@ -1,0 +1,140 @@
+"""GPU: hipGraph replay of the launch sequence and the autotuner.
+
+A replayed graph must give exactly the eager result, call after call, with
+the program's own eager work (other decompositions, synchronisations) in
+between.  A tuned launch configuration changes the schedule only: FP64
+extrema / IMF counts stay identical and the ensemble mean moves by at most
+re-association rounding.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from eemd4cuda_amd import _lib
+    return _lib.engine()
+
+
+def _params(fp64, B=256, S=30, max_imf=6):
+    from eemd4cuda_amd import _lib
+    p = _lib.default_params()
+    p.fp64 = fp64
+    p.max_imf = max_imf
+    p.block = B
+    p.halo = S if fp64 else 14
+    p.seed = 5
+    return p
+
+
+def _batch(seed, lens):
+    rng = np.random.default_rng(seed)
+    xs = [np.sin(np.arange(n) * rng.uniform(0.01, 0.2)) + 0.3 * rng.standard_normal(n)
+          for n in lens]
+    offs = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    sig = np.array([0.2 * np.std(v) for v in xs])
+    return np.concatenate(xs), offs, sig
+
+
+@pytest.mark.parametrize("fp64", [1, 0])
+def test_graph_replays_equal_eager(eng, fp64):
+    import torch
+    x, offs, sig = _batch(1, [3000, 7000, 12000])
+    xo, offo, sigo = _batch(2, [2500, 4000])
+    E = 6
+    print("case: graph fp64=%d" % fp64, flush=True)
+    ref = eng.run_batch(x, offs, E, sig, _params(fp64))
+    cap0, _ = eng.graph_stats()
+    for rep in range(3):
+        g = eng.run_batch(x, offs, E, sig, _params(fp64), graph=True)
+        cap, _ = eng.graph_stats()
+        assert cap == cap0 + 1  # captured once, replayed afterwards
+        assert g["timing"]["mean_ms"] == 0.0  # served by the graph
+        assert np.array_equal(g["avg"], ref["avg"]), rep
+        assert np.array_equal(g["resid"], ref["resid"]), rep
+        assert np.array_equal(g["nmode"], ref["nmode"]), rep
+        assert np.array_equal(g["nsift"], ref["nsift"]), rep
+        # the program's own eager work between replays (smaller batch: same buffers)
+        o = eng.run_batch(xo, offo, E, sigo, _params(fp64))
+        assert np.all(o["n_modes"] > 0)
+        torch.cuda.synchronize()
+    # a different batch through the graph path re-captures and is still exact
+    o_ref = eng.run_batch(xo, offo, E, sigo, _params(fp64))
+    o_g = eng.run_batch(xo, offo, E, sigo, _params(fp64), graph=True)
+    assert np.array_equal(o_g["avg"], o_ref["avg"])
+
+
+def test_graph_device_buffers_replay(eng):
+    import torch
+    from eemd4cuda_amd import _lib
+    x, offs, sig = _batch(3, [5000, 9000])
+    E, mi = 4, 6
+    dev = torch.device("cuda", 0)
+    xd = torch.from_numpy(x).to(dev)
+    outs = []
+    for graph in (False, True, True, True):
+        avg = torch.full((mi * len(x),), 7.0, dtype=torch.float64, device=dev)
+        res = torch.zeros(len(x), dtype=torch.float64, device=dev)
+        nm = torch.zeros(2, dtype=torch.int32, device=dev)
+        nmode = torch.zeros(2 * E, dtype=torch.int32, device=dev)
+        nsift = torch.zeros(2 * E, dtype=torch.int32, device=dev)
+        p = _params(1, max_imf=mi)
+        p.flags = _lib.FLAG_GRAPH if graph else 0
+        print("case: device graph", graph, flush=True)
+        eng.run_batch_dev(xd.data_ptr(), offs, E, sig, p, avg.data_ptr(), res.data_ptr(),
+                          nm.data_ptr(), nmode.data_ptr(), nsift.data_ptr())
+        torch.cuda.synchronize()
+        outs.append((avg.cpu().numpy(), res.cpu().numpy(), nmode.cpu().numpy()))
+    # fresh output tensors each call: the graph is re-captured for new addresses
+    for o in outs[1:]:
+        for a, b in zip(o, outs[0]):
+            assert np.array_equal(a, b)
+
+
+def _dev_run(eng, xd, offs, E, sig, p, mi):
+    import torch
+    dev = xd.device
+    n = xd.numel()
+    nsig = len(offs) - 1
+    avg = torch.zeros(mi * n, dtype=torch.float64, device=dev)
+    res = torch.zeros(n, dtype=torch.float64, device=dev)
+    nm = torch.zeros(nsig, dtype=torch.int32, device=dev)
+    nmode = torch.zeros(nsig * E, dtype=torch.int32, device=dev)
+    nsift = torch.zeros(nsig * E, dtype=torch.int32, device=dev)
+    eng.run_batch_dev(xd.data_ptr(), offs, E, sig, p, avg.data_ptr(), res.data_ptr(),
+                      nm.data_ptr(), nmode.data_ptr(), nsift.data_ptr())
+    torch.cuda.synchronize()
+    tm = eng.last_timing()
+    return dict(avg=avg.cpu().numpy(), resid=res.cpu().numpy(), nmode=nmode.cpu().numpy(),
+                nsift=nsift.cpu().numpy(), timing=tm)
+
+
+@pytest.mark.parametrize("fp64", [1, 0])
+def test_autotune_choice_is_applied_and_exact(eng, fp64):
+    import torch
+    x, offs, sig = _batch(4, [4000, 16000, 40000, 9000])
+    E, mi = 12, 6
+    xd = torch.from_numpy(x).to(torch.device("cuda", 0))
+    p = _params(fp64, B=512, max_imf=mi)
+    print("case: autotune fp64=%d" % fp64, flush=True)
+    base = _dev_run(eng, xd, offs, E, sig, p, mi)
+    best, ms = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
+    assert best.threads in (256, 512, 1024)
+    assert 0 <= best.members_per_task <= E  # 0: the automatic per-signal rule
+    assert ms > 0
+    if not fp64:
+        assert best.wave_windows != 0
+    # explicit best and the cached choice (all-automatic parameters) agree bit for bit
+    exp = _dev_run(eng, xd, offs, E, sig, best, mi)
+    cached = _dev_run(eng, xd, offs, E, sig, p, mi)
+    assert cached["timing"]["threads"] == best.threads
+    assert np.array_equal(exp["avg"], cached["avg"])
+    assert np.array_equal(exp["nsift"], cached["nsift"])
+    if fp64:
+        # same members, same extrema: only the ensemble sum is re-associated
+        assert np.array_equal(exp["nmode"], base["nmode"])
+        assert np.array_equal(exp["nsift"], base["nsift"])
+        scale = np.max(np.abs(base["avg"]))
+        assert np.max(np.abs(exp["avg"] - base["avg"])) <= 1e-13 * scale
