@@ -16,6 +16,11 @@ reference's CPU path, restated in numpy (oracle/emd_np.py), on a bounded
 sample of members of the same workload, one member per host core.
 
     python bench.py [--steps 5] [--warmup 1] [--workload audio_mix|three_tone|uneven|chirp_scale]
+                    [--learned] [--compare]
+
+The uneven batch also reports plain per-signal launches against the batched,
+graph-replayed and autotuned launch; the chirp/AM set also reports learned
+ends (throughput cost and edge-region error against mirror ends).
 """
 from __future__ import annotations
 
@@ -32,6 +37,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg):
+    """Progress on stderr (stdout carries only the JSON result line)."""
+    print("[bench %.0fs] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
+
+_T0 = time.perf_counter()
 BYTES_PER_SAMPLE_SIFT = {"f32": 10.4, "f64": 19.2}
 
 
@@ -158,6 +171,27 @@ def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0):
                               frac=achieved / HBM_PEAK_GBS, traffic=None)), r
 
 
+def learned_leg(wl, steps, warmup, halo, mirror_value):
+    """FP32 throughput with learned ends (packaged model) on the same
+    workload, its cost against mirror ends, and the edge-region IMF error of
+    mirror vs learned ends on 256 held-out chirp/AM windows (FP64)."""
+    import torch
+    from eemd4cuda_amd import _lib, learned
+    params = learned.load(learned.default_model_path())
+    learned.install(params)
+    r = DeviceRun(wl, "f32", halo)
+    r.p.end_mode = _lib.END_MODES["learned"]
+    for _ in range(warmup):
+        r.step()
+    torch.cuda.synchronize()
+    t = _timed(r, steps, r.step)
+    ss = r.sample_sifts()
+    v = ss / t
+    ev = learned.edge_error(params, n_windows=256)
+    return dict(value=v, ms_per_step=t * 1e3, cost_vs_mirror=1.0 - v / mirror_value,
+                edge_error=ev, model="eemd4cuda_amd/boundary_model.npz")
+
+
 def _timed(r, steps, fn):
     import torch
     torch.cuda.synchronize()
@@ -264,6 +298,8 @@ def main():
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
+    ap.add_argument("--learned", action="store_true",
+                    help="also time learned ends (packaged model) and report its edge error")
     ap.add_argument("--compare", action="store_true",
                     help="also time plain per-signal launches, graph replay and the autotuned "
                          "configuration (default for the uneven batch)")
@@ -279,6 +315,7 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
+    _log("building workload %s" % a.workload)
     wl = workload(a.workload)
     if a.block:
         wl["B"] = a.block
@@ -286,8 +323,10 @@ def main():
         wl["E"] = a.E
     if a.max_imf:
         wl["max_imf"] = a.max_imf
+    _log("timing f32")
     f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads, mpt=a.mpt,
                          wave_windows=a.wave_windows)
+    _log("f32 %.1f ms per step" % f32["ms_per_step"])
     out = {
         "metric": "ensemble sample-sifts/s",
         "value": f32["value"],
@@ -313,6 +352,7 @@ def main():
         r32.step(phases=True)
         out["phases_ms_f32"] = dict(zip(("solve", "pass", "fixup", "other"), r32.eng.last_phases()))
     if not a.no_fp64:
+        _log("timing f64")
         f64, r64 = time_mode(wl, "f64", max(1, a.steps // 2), a.warmup, halo=30,
                              threads=a.threads, mpt=a.mpt)
         out["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "sift_kernel_ms", "roofline")}
@@ -325,13 +365,18 @@ def main():
                          for k in range(m)])
         out["fp32_imf_rel_l2_vs_fp64"] = errs[0] if len(errs) == 1 else \
             [float(np.max([e[k] for e in errs if len(e) > k])) for k in range(max(map(len, errs)))]
+    if a.learned or a.workload == "chirp_scale":
+        _log("timing f32 with learned ends")
+        out["learned_f32"] = learned_leg(wl, a.steps, a.warmup, a.halo32, f32["value"])
     if a.compare or a.workload == "uneven":
+        _log("comparators")
         out["comparators_f32"] = comparators(wl, "f32", max(1, a.steps // 2), a.halo32)
     prof = ROOT / "profiles" / ("traffic_%s.json" % a.workload)
     if prof.exists():
         tr = json.loads(prof.read_text())
         out["roofline"]["traffic"] = tr.get("f32_bytes_per_launch")
     if not a.no_cpu and len(wl["xs"]) >= 1:
+        _log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_cores)
         if a.workload in ("audio_mix", "three_tone"):
             out["cpu_baseline_c"] = cpu_baseline_c(wl, a.cpu_cores)

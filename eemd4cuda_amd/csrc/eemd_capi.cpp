@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -15,10 +16,11 @@
 
 #include "../../include/eemd_mi355x.h"
 #include "eemd_common.h"
+#include "eemd_mlp.h"
 
 namespace eemd {
 hipError_t launch_sift(const SiftArgs& A, int fp64, int nt, int grid, hipStream_t st);
-hipError_t sift_occupancy(int fp64, int nt, int* blocks_per_cu);
+hipError_t sift_occupancy(int fp64, int nt, int end_mode, int* blocks_per_cu);
 hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start,
                        const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
                        int max_imf, const double* x, double* avg, double* resid,
@@ -108,6 +110,8 @@ struct eemd_ctx {
   DevBuf hx, hnoise, havg, hres, hnm, hnmode, hnsift, hpm, hmi, htr;
   // outputs of the autotuner's timed runs
   DevBuf tx, tavg, tres, tnm, tnmode, tnsift, tpm;
+  DevBuf mlpw;  // BF16 boundary model (EEMD_END_LEARNED)
+  bool have_model = false;
   GraphCache graph;
   std::vector<TuneEntry> tuned;
   double last_sift_ms = 0.0, last_mean_ms = 0.0;
@@ -186,8 +190,8 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
     return fail(c, EEMD_ERR_BAD_ARG, "max_imf must be in [1, 64]");
   if (p->max_sifts < 0) return fail(c, EEMD_ERR_BAD_ARG, "max_sifts must be >= 0");
   if (p->end_mode < 0 || p->end_mode > 2) return fail(c, EEMD_ERR_BAD_ARG, "bad end_mode");
-  if (p->end_mode == EEMD_END_LEARNED)
-    return fail(c, EEMD_ERR_BAD_ARG, "learned boundary needs a model (eemd_run_learned)");
+  if (p->end_mode == EEMD_END_LEARNED && !c->have_model)
+    return fail(c, EEMD_ERR_BAD_ARG, "learned boundary needs a model (eemd_set_boundary_model)");
   if (p->halo < 0) return fail(c, EEMD_ERR_BAD_ARG, "halo must be >= 0");
   if (p->threads != 0 && p->threads != 256 && p->threads != 512 && p->threads != 1024)
     return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 256, 512 or 1024");
@@ -235,7 +239,7 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   P.wave_windows = want_ww == 0 ? 8 * nw : (want_ww < 0 ? -1 : want_ww);
 
   int bpc = 1;
-  hipError_t oe = sift_occupancy(p->fp64, P.threads, &bpc);
+  hipError_t oe = sift_occupancy(p->fp64, P.threads, p->end_mode, &bpc);
   if (oe != hipSuccess) return hip_fail(c, oe, "occupancy query");
   if (bpc < 1) bpc = 1;
   const int64_t max_slots = (int64_t)bpc * c->ncu;
@@ -420,7 +424,7 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.cap = P.cap;
   A.rows = P.rows;
   A.cols = P.cols;
-  A.mlp = nullptr;
+  A.mlp = (p->end_mode == EEMD_END_LEARNED) ? c->mlpw.p : nullptr;
   A.nslots = P.grid;
   A.phase = nullptr;
   A.wave_windows = P.wave_windows;
@@ -551,7 +555,8 @@ int eemd_ctx_destroy(eemd_ctx* c) {
   c->desc.release();
   for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->queue, &c->phase, &c->hx, &c->hnoise,
                     &c->havg, &c->hres, &c->hnm, &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr,
-                    &c->tx, &c->tavg, &c->tres, &c->tnm, &c->tnmode, &c->tnsift, &c->tpm})
+                    &c->tx, &c->tavg, &c->tres, &c->tnm, &c->tnmode, &c->tnsift, &c->tpm,
+                    &c->mlpw})
     b->release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -676,7 +681,7 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
     int r = make_plan(c, offsets, nsig, E, sigmas, noise, &q, false, P);
     if (r) return r;
     int bpc = 1;
-    HIPCHK(c, sift_occupancy(q.fp64, P.threads, &bpc), "occupancy query");
+    HIPCHK(c, sift_occupancy(q.fp64, P.threads, q.end_mode, &bpc), "occupancy query");
     slots = std::max(1, bpc) * c->ncu;
     ms = 1e300;
     for (int k = 0; k < reps; ++k) {
@@ -713,7 +718,7 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
   // idle or need far more accumulator memory than the automatic rule
   {
     int bpc = 1;
-    HIPCHK(c, sift_occupancy(prm->fp64, bt, &bpc), "occupancy query");
+    HIPCHK(c, sift_occupancy(prm->fp64, bt, prm->end_mode, &bpc), "occupancy query");
     const int sl = std::max(1, bpc) * c->ncu;
     const int64_t auto_acc = acc_bytes(bt, 0);
     std::vector<int> cand;
@@ -789,6 +794,22 @@ int eemd_last_timing(eemd_ctx* c, double* sift_ms, double* mean_ms, int32_t* thr
   if (threads) *threads = c->last_threads;
   if (grid) *grid = c->last_grid;
   if (ntasks) *ntasks = c->last_ntask;
+  return EEMD_OK;
+}
+
+int eemd_set_boundary_model(eemd_ctx* c, const float* params) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  if (!params) return fail(c, EEMD_ERR_BAD_ARG, "params is NULL");
+  for (int i = 0; i < EEMD_MLP_NPARAM; ++i)
+    if (!std::isfinite(params[i])) return fail(c, EEMD_ERR_BAD_ARG, "non-finite model parameter");
+  std::vector<uint16_t> w(EEMD_MLP_NPARAM);
+  for (int i = 0; i < EEMD_MLP_NPARAM; ++i) w[i] = f32_to_bf16(params[i]);
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  HIPCHK(c, c->mlpw.ensure(sizeof(uint16_t) * EEMD_MLP_NPARAM), "model alloc");
+  HIPCHK(c, hipStreamSynchronize(c->stream), "model upload");
+  HIPCHK(c, hipMemcpy(c->mlpw.p, w.data(), sizeof(uint16_t) * EEMD_MLP_NPARAM,
+                      hipMemcpyHostToDevice), "model upload");
+  c->have_model = true;
   return EEMD_OK;
 }
 

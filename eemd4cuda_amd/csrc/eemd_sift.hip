@@ -35,6 +35,7 @@
 #include <type_traits>
 
 #include "eemd_common.h"
+#include "eemd_mlp.h"
 
 namespace eemd {
 
@@ -60,6 +61,14 @@ struct Shared {
   int32_t pm[MAX_IMF_CAP];
   int32_t task;
   double sd;
+};
+
+// Staging of the learned end extension (both ends of one sift).
+struct LearnedLDS {
+  double ls[2][2][MLP_CTX];  // [end][env][0]: distance of the nearest knot from the end
+  __attribute__((aligned(16))) float x[2][MLP_IN];       // model input per end
+  __attribute__((aligned(16))) float h[2][2][MLP_HID];   // hidden activations (ping-pong)
+  float g[2], mu[2], sd[2];  // gap scale, value mean and spread per end
 };
 
 // ------------------------------------------------------------------------
@@ -849,6 +858,136 @@ __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, in
   }
 }
 
+// Learned end extension (all threads of the workgroup; call after
+// setup_ends put the mirror extension in place, which stays when an envelope
+// has fewer than MLP_CTX extrema).  For each end, the MLP_CTX extrema nearest
+// to it on both envelopes give the model input:
+//   gaps   d_0 = distance of the nearest extremum from the end sample,
+//          d_i = distance between the i-th and (i-1)-th nearest, as d/G - 1
+//          with G the mean of the 30 inter-extremum gaps of both envelopes;
+//   values (y - mu) / sd over the 32 values of both envelopes;
+// the sums behind G, mu and sd are xor-butterfly trees over the end's 32
+// [env][i] slots (partner offsets 16, 8, 4, 2, 1).
+// Per envelope the output o[0..3] gives gaps (first: from the end sample,
+// G exp(o) - 1 >= 0, kept >= 2 from the nearest extremum; then G exp(o) >= 2
+// apart) and o[4..7] values mu + sd o of MLP_PRED virtual extrema beyond the
+// end.  The outermost virtual knot on each side carries M = 0.
+template <typename T, int NW, int NT>
+__device__ __forceinline__ void learned_ends(Shared<T, NW>& S, LearnedLDS& L, const Mem<T>& M,
+                                             int cur, const uint16_t* w) {
+  const int tid = (int)threadIdx.x;
+  if (S.p[0] < MLP_CTX || S.p[1] < MLP_CTX) return;  // workgroup-uniform
+  if (tid < 4 * MLP_CTX) {
+    // wave 0, lane = [end][env][i]: gather, scale statistics and features.
+    // The sums over an end's 32 lanes use a fixed xor-butterfly tree
+    // (partner offsets 16, 8, 4, 2, 1), so every lane holds the same value.
+    const int end = tid >> 5, ev = (tid >> 4) & 1, i = tid & 15;
+    const int p = S.p[ev];
+    int wh = end ? NW - 1 : 0;
+    const int64_t slot = real_slot(S, M, cur, ev, end ? p - 1 - i : i, wh);
+    const T* r = rec(M, cur, ev, slot);
+    const double t = (double)r[0];
+    const double ls = end ? (double)(M.n - 1) - t : t;
+    const float yf = (float)(double)r[1];
+    const double lsp = __shfl_up(ls, 1);
+    const float gap = (float)(i == 0 ? ls : ls - lsp);
+    float gs = (i == 0) ? 0.0f : gap, ys = yf;
+#pragma unroll
+    for (int k = 16; k >= 1; k >>= 1) {
+      gs += __shfl_xor(gs, k);
+      ys += __shfl_xor(ys, k);
+    }
+    const float G = gs / (float)(2 * (MLP_CTX - 1));
+    const float mu = ys / (float)(2 * MLP_CTX);
+    const float dv = yf - mu;
+    float vs = dv * dv;
+#pragma unroll
+    for (int k = 16; k >= 1; k >>= 1) vs += __shfl_xor(vs, k);
+    const float sd = fmaxf(sqrtf(vs / (float)(2 * MLP_CTX)), 1e-30f);
+    L.x[end][ev * 32 + i] = gap / G - 1.0f;
+    L.x[end][ev * 32 + 16 + i] = (yf - mu) / sd;
+    if (i == 0) L.ls[end][ev][0] = ls;
+    if ((tid & 31) == 0) {
+      L.g[end] = G;
+      L.mu[end] = mu;
+      L.sd[end] = sd;
+    }
+  }
+  __syncthreads();
+  // Layers: 512 work items [end][neuron][half]; in each wave, lanes 0-31
+  // run the first-half chains and lanes 32-63 the second-half chains of the
+  // same 32 neurons, joined with one cross-half exchange.  The 32 outputs of
+  // the last layer stay in wave 0's lanes 0-31 ([end][output]).
+  float out = 0.0f;
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    const int nin = l == 0 ? MLP_IN : MLP_HID;
+    const int nout = l == 3 ? MLP_OUT : MLP_HID;
+    const int wrow = l == 0 ? MLP_LDS_W1 : l == 1 ? MLP_LDS_W2 : l == 2 ? MLP_LDS_W3 : MLP_LDS_W4;
+    const int brow = l == 0 ? MLP_LDS_B1 : l == 1 ? MLP_LDS_B2 : l == 2 ? MLP_LDS_B3 : MLP_LDS_B4;
+    const int items = 2 * nout * 2;
+    for (int it = tid; it < ((items + WAVE - 1) / WAVE) * WAVE; it += NT) {
+      const int wv = it >> 6, ln = it & 63, half = ln >> 5;
+      const int q = wv * 32 + (ln & 31);  // [end][neuron]
+      const int end = q / nout, jn = q - end * nout;
+      const bool live = q < 2 * nout;
+      float part = 0.0f;
+      if (live) {
+        const float* in = l == 0 ? L.x[end] : L.h[(l - 1) & 1][end];
+        const uint16_t* row = w + wrow + jn * (nin + MLP_LDS_PAD) + half * (nin / 2);
+        const float acc0 = half ? 0.0f : bf16_to_f32(w[brow + jn]);
+        part = mlp_half_row(row, acc0, in + half * (nin / 2), nin / 2);
+      }
+      const float z = part + __shfl_xor(part, 32);
+      if (l < 3) {
+        if (live && half == 0) L.h[l & 1][end][jn] = mlp_gelu(z);
+      } else {
+        out = z;
+      }
+    }
+    if (l < 3) __syncthreads();
+  }
+  if (tid < WAVE) {
+    // wave 0: one virtual extremum per lane [end][env][k] (lanes 0-15); the
+    // gaps of its envelope and end are recomputed in order, so the positions
+    // equal the sequential construction bit for bit
+    const int end = (tid >> 3) & 1, ev = (tid >> 2) & 1, kk = tid & 3;
+    const int ob = end * MLP_OUT + ev * 8;  // lane holding o[0] of this envelope
+    float o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = __shfl(out, ob + k);
+    if (tid < 2 * 2 * MLP_PRED) {
+      const float G = L.g[end], mu = L.mu[end], sd = L.sd[end];
+      const double dir = end ? 1.0 : -1.0;
+      const double e_end = end ? (double)(M.n - 1) : 0.0;
+      const double t_near = e_end - dir * L.ls[end][ev][0];
+      const float d0 = fmaxf(G * mlp_exp(fminf(fmaxf(o[0], -4.0f), 4.0f)) - 1.0f, 0.0f);
+      double q = e_end + dir * (double)d0;
+      if (dir * (q - t_near) < 2.0) q = t_near + dir * 2.0;
+      for (int k = 1; k <= kk; ++k) {
+        const float gk = fmaxf(G * mlp_exp(fminf(fmaxf(o[k], -4.0f), 4.0f)), 2.0f);
+        q = q + dir * (double)gk;
+      }
+      float ov = o[4];
+      ov = kk == 1 ? o[5] : ov;
+      ov = kk == 2 ? o[6] : ov;
+      ov = kk == 3 ? o[7] : ov;
+      // left end: slots in time order, outermost first; right end: nearest first
+      const int slot = end ? MAX_VIRT + kk : MLP_PRED - 1 - kk;
+      T* r = rec(M, cur, ev, M.vbase + slot);
+      r[0] = (T)q;
+      r[1] = (T)(mu + sd * ov);
+      r[2] = (T)0;
+      S.vt[ev][slot] = (T)q;
+      if (kk == 0) {
+        if (end) S.nr[ev] = MLP_PRED;
+        else S.nl[ev] = MLP_PRED;
+        S.P[ev] = S.p[ev] + 2 * MLP_PRED;
+      }
+    }
+  }
+}
+
 template <typename T, int NW>
 __device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int cur, int env,
                                         int code, double val) {
@@ -1302,6 +1441,8 @@ template <typename T, int NT>
 __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   constexpr int NW = NT / WAVE;
   __shared__ Shared<T, NW> S;
+  __shared__ LearnedLDS LL;
+  extern __shared__ __align__(16) uint16_t model_lds[];  // learned mode: MLP_LDS_BYTES
   const int tid = (int)threadIdx.x;
 
   char* scr = A.scratch + (int64_t)blockIdx.x * A.scratch_stride;
@@ -1328,6 +1469,25 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
     const uint64_t t_ = clk();            \
     ph[k] += t_ - tc;                     \
     tc = t_;                              \
+  }
+
+  if (A.end_mode == END_LEARNED) {
+    // the BF16 model stays in LDS for the workgroup's lifetime, one row per
+    // neuron (global layout W[in][out] read coalesced)
+    const uint16_t* g = reinterpret_cast<const uint16_t*>(A.mlp);
+    const int gw[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4};
+    const int gb[4] = {MLP_B1, MLP_B2, MLP_B3, MLP_B4};
+    const int lw[4] = {MLP_LDS_W1, MLP_LDS_W2, MLP_LDS_W3, MLP_LDS_W4};
+    const int lb[4] = {MLP_LDS_B1, MLP_LDS_B2, MLP_LDS_B3, MLP_LDS_B4};
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+      const int nin = l == 0 ? MLP_IN : MLP_HID, nout = l == 3 ? MLP_OUT : MLP_HID;
+      for (int k = tid; k < nin * nout; k += NT) {
+        const int i = k / nout, j = k - i * nout;
+        model_lds[lw[l] + j * (nin + MLP_LDS_PAD) + i] = g[gw[l] + k];
+      }
+      for (int j = tid; j < nout; j += NT) model_lds[lb[l] + j] = g[gb[l] + j];
+    }
   }
 
   for (;;) {
@@ -1383,7 +1543,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           __syncthreads();
           const int nxt = cur ^ 1;
           if (tid == 0) {
-            setup_ends(S, M, cur, A.end_mode);
+            setup_ends(S, M, cur, A.end_mode == END_LEARNED ? (int)END_MIRROR : A.end_mode);
             S.hsh[nxt][0] = S.hsh[nxt][1] = 0ull;
             if (trace) {
               uint64_t* tr = A.trace + (((int64_t)unit * A.max_imf + m) * A.max_sifts + s) * 4;
@@ -1394,6 +1554,10 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
             }
           }
           __syncthreads();
+          if (A.end_mode == END_LEARNED) {
+            learned_ends<T, NW, NT>(S, LL, M, cur, model_lds);
+            __syncthreads();
+          }
           PHASE(3);
           solve(S, M, cur, A.B, A.S, A.wave_windows);
           __syncthreads();
@@ -1548,9 +1712,27 @@ __global__ void noise_kernel(uint64_t seed, uint32_t sig, uint32_t m0, int E, in
 // ------------------------------------------------------------------------
 // host-side launchers (called from the C-ABI)
 // ------------------------------------------------------------------------
+// Dynamic LDS of a launch: the BF16 boundary model in learned mode.
+static size_t sift_dyn_lds(int end_mode) {
+  return end_mode == END_LEARNED ? MLP_LDS_BYTES : 0;
+}
+
+template <typename T, int NT>
+static hipError_t allow_model_lds() {
+  static const hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(sift_kernel<T, NT>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, (int)MLP_LDS_BYTES);
+  return e;
+}
+
 template <typename T, int NT>
 static hipError_t launch_sift_t(const SiftArgs& A, int grid, hipStream_t st) {
-  hipLaunchKernelGGL((sift_kernel<T, NT>), dim3(grid), dim3(NT), 0, st, A);
+  const size_t lds = sift_dyn_lds(A.end_mode);
+  if (lds) {
+    const hipError_t e = allow_model_lds<T, NT>();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL((sift_kernel<T, NT>), dim3(grid), dim3(NT), lds, st, A);
   return hipGetLastError();
 }
 
@@ -1571,7 +1753,7 @@ hipError_t launch_sift(const SiftArgs& A, int fp64, int nt, int grid, hipStream_
   return hipErrorInvalidValue;
 }
 
-hipError_t sift_occupancy(int fp64, int nt, int* blocks_per_cu) {
+hipError_t sift_occupancy(int fp64, int nt, int end_mode, int* blocks_per_cu) {
   const void* f = nullptr;
   if (fp64) {
     if (nt == 256) f = (const void*)sift_kernel<double, 256>;
@@ -1583,7 +1765,17 @@ hipError_t sift_occupancy(int fp64, int nt, int* blocks_per_cu) {
     if (nt == 1024) f = (const void*)sift_kernel<float, 1024>;
   }
   if (!f) return hipErrorInvalidValue;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, nt, 0);
+  const size_t lds = sift_dyn_lds(end_mode);
+  if (lds) {
+    const hipError_t e = fp64 ? (nt == 256 ? allow_model_lds<double, 256>()
+                                 : nt == 512 ? allow_model_lds<double, 512>()
+                                             : allow_model_lds<double, 1024>())
+                              : (nt == 256 ? allow_model_lds<float, 256>()
+                                 : nt == 512 ? allow_model_lds<float, 512>()
+                                             : allow_model_lds<float, 1024>());
+    if (e != hipSuccess) return e;
+  }
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, nt, lds);
 }
 
 hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start,

@@ -163,6 +163,47 @@ int eemd_graph_stats(eemd_ctx* ctx, int64_t* captures, int64_t* launches);
  * dbg_serial_split_v0.py:25-42). */
 int eemd_last_phases(eemd_ctx* ctx, double* ms4);
 
+/* ------------------------------------------------------------------------
+ * Learned boundary extension.  No counterpart in the reference: its GPU
+ * kernels use natural ends and its CPU oracle the mirror rule
+ * (emd_ref.py:92-95), which EEMD_END_LEARNED falls back to when an envelope
+ * has fewer than 16 extrema.
+ *
+ * Model: MLP 64 -> 128 -> 128 -> 128 -> 16, GELU (tanh form), BF16
+ * parameters.  Input per end: the 16 extrema nearest to the end on both
+ * envelopes (gaps scaled by their mean, values standardised); output: 4
+ * virtual extrema per envelope beyond the end.  Parameters are FP32, packed
+ * per layer as W[in][out] then bias[out] (EEMD_MLP_NPARAM floats).
+ * ------------------------------------------------------------------------ */
+#define EEMD_MLP_NPARAM 43408
+
+typedef struct eemd_mlp eemd_mlp;
+
+/* Install the boundary model used by EEMD_END_LEARNED calls on this context
+ * (parameters rounded to BF16). */
+int eemd_set_boundary_model(eemd_ctx* ctx, const float* params);
+
+/* Training state on one device: FP32 master parameters, Adam moments, BF16
+ * copy; LeCun-uniform initial weights from `seed`, zero biases. */
+int eemd_mlp_create(int device, uint64_t seed, eemd_mlp** out);
+int eemd_mlp_destroy(eemd_mlp* m);
+const char* eemd_mlp_last_error(const eemd_mlp* m);
+/* Host FP32 parameters in / out (set also resets the Adam state). */
+int eemd_mlp_set(eemd_mlp* m, const float* params);
+int eemd_mlp_get(eemd_mlp* m, float* params);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the mean squared error of a
+ * batch (multiple of 16): x [batch][64], y [batch][16] device memory;
+ * loss (host, optional) is the batch loss before the step. */
+int eemd_mlp_step(eemd_mlp* m, const float* x, const float* y, int32_t batch, float lr,
+                  float* loss);
+/* Gradient of the batch loss w.r.t. the parameters (host, EEMD_MLP_NPARAM)
+ * without updating them. */
+int eemd_mlp_grad(eemd_mlp* m, const float* x, const float* y, int32_t batch, float* grad,
+                  float* loss);
+/* Inference with the arithmetic of the in-kernel extension: x [batch][64],
+ * y [batch][16], device memory. */
+int eemd_mlp_infer(eemd_mlp* m, const float* x, int32_t batch, float* y);
+
 #ifdef __cplusplus
 }
 #endif

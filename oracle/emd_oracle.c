@@ -19,6 +19,9 @@
  *   emd_ref.py:141-142 np.sum (8192-element blocks, pairwise inside)  -> orc_pairwise_sum()
  * plus the counter-based noise the GPU path generates in-kernel:
  *   Philox4x32-10 (Salmon et al., SC'11) + Box-Muller                -> orc_philox4x32_10(), orc_normals()
+ * and the engine's learned end extension (no reference counterpart): the
+ * boundary MLP's exact FP32 inference (orc_mlp_infer), its virtual-extrema
+ * rule (orc_learned_virtual) and envelopes with those ends (end mode 2).
  *
  * Deliberate arithmetic choices shared with the HIP FP64 path:
  *   - a**3 (emd_ref.py:75) is numpy's SIMD pow there; here it is the cube
@@ -40,6 +43,7 @@
 
 #define ORC_END_NATURAL 0
 #define ORC_END_MIRROR 1
+#define ORC_END_LEARNED 2
 
 /* ------------------------------------------------------------------ */
 /* small helpers                                                        */
@@ -221,10 +225,14 @@ typedef struct {
   double* r;
   double* x;
   int64_t* k;
+  double* Tx;  /* learned ends: knots of both envelopes at once */
+  double* Yx;
+  double* Tn;
+  double* Yn;
 } orc_work;
 
 static int orc_work_alloc(orc_work* w, int64_t N) {
-  size_t m = (size_t)(N + 4);
+  size_t m = (size_t)(N + 16);
   w->T = (double*)malloc(sizeof(double) * m);
   w->Y = (double*)malloc(sizeof(double) * m);
   w->M = (double*)malloc(sizeof(double) * m);
@@ -234,18 +242,37 @@ static int orc_work_alloc(orc_work* w, int64_t N) {
   w->r = (double*)malloc(sizeof(double) * m);
   w->x = (double*)malloc(sizeof(double) * m);
   w->k = (int64_t*)malloc(sizeof(int64_t) * m);
-  return (w->T && w->Y && w->M && w->d && w->l && w->u && w->r && w->x && w->k) ? 0 : -1;
+  w->Tx = (double*)malloc(sizeof(double) * m);
+  w->Yx = (double*)malloc(sizeof(double) * m);
+  w->Tn = (double*)malloc(sizeof(double) * m);
+  w->Yn = (double*)malloc(sizeof(double) * m);
+  return (w->T && w->Y && w->M && w->d && w->l && w->u && w->r && w->x && w->k && w->Tx &&
+          w->Yx && w->Tn && w->Yn) ? 0 : -1;
 }
 
 static void orc_work_free(orc_work* w) {
   free(w->T); free(w->Y); free(w->M); free(w->d); free(w->l);
   free(w->u); free(w->r); free(w->x); free(w->k);
+  free(w->Tx); free(w->Yx); free(w->Tn); free(w->Yn);
 }
 
 static uint64_t orc_hash_idx(const int64_t* k, int64_t p) {
   uint64_t s = 0;
   for (int64_t i = 0; i < p; i++) s += orc_mix64((uint64_t)k[i]);
   return s;
+}
+
+/* Natural spline through the P extended knots in w->T / w->Y (M = 0 at
+ * both outermost knots), evaluated on 0..N-1. */
+static void orc_spline_from_knots(orc_work* w, int64_t P, int64_t N, int64_t B, int64_t S,
+                                  double* env) {
+  int64_t n = P - 2;
+  for (int64_t q = 0; q < n; q++) orc_row(w->T, w->Y, n, q, &w->d[q], &w->l[q], &w->u[q], &w->r[q]);
+  orc_dsbt(w->d, w->l, w->u, w->r, n, B, S, w->x);
+  w->M[0] = 0.0;
+  w->M[P - 1] = 0.0;
+  for (int64_t q = 0; q < n; q++) w->M[q + 1] = w->x[q];
+  orc_spline_eval(w->T, w->Y, w->M, P, N, env);
 }
 
 static int64_t orc_envelope_w(const double* h, int64_t N, int want_max, int end,
@@ -278,13 +305,7 @@ static int64_t orc_envelope_w(const double* h, int64_t N, int want_max, int end,
     w->T[p + 1] = w->T[p] + dt1;
     w->Y[p + 1] = w->Y[p - 1];
   }
-  int64_t n = P - 2;
-  for (int64_t q = 0; q < n; q++) orc_row(w->T, w->Y, n, q, &w->d[q], &w->l[q], &w->u[q], &w->r[q]);
-  orc_dsbt(w->d, w->l, w->u, w->r, n, B, S, w->x);
-  w->M[0] = 0.0;
-  w->M[P - 1] = 0.0;
-  for (int64_t q = 0; q < n; q++) w->M[q + 1] = w->x[q];
-  orc_spline_eval(w->T, w->Y, w->M, P, N, env);
+  orc_spline_from_knots(w, P, N, B, S, env);
   return p;
 }
 
@@ -315,6 +336,258 @@ int64_t orc_spline_knots(const double* h, int64_t N, int want_max, int end, int6
   free(env);
   orc_work_free(&w);
   return P;
+}
+
+/* ------------------------------------------------------------------ */
+/* Learned end extension (the engine's EEMD_END_LEARNED; no reference    */
+/* counterpart).  Model: 64 -> 128 -> 128 -> 128 -> 16, GELU, BF16       */
+/* parameters packed per layer as W[in][out] then bias[out].            */
+/* Inference arithmetic: each neuron two FP32 fma chains in index order */
+/* (first half from the bias, second from 0) added; GELU in tanh form;  */
+/* exp by Cody-Waite                                                    */
+/* reduction and a degree-7 Taylor polynomial.                          */
+/* ------------------------------------------------------------------ */
+#define ORC_MLP_IN 64
+#define ORC_MLP_HID 128
+#define ORC_MLP_OUT 16
+#define ORC_CTX 16
+#define ORC_PRED 4
+#define ORC_W1 0
+#define ORC_B1 (ORC_W1 + ORC_MLP_IN * ORC_MLP_HID)
+#define ORC_W2 (ORC_B1 + ORC_MLP_HID)
+#define ORC_B2 (ORC_W2 + ORC_MLP_HID * ORC_MLP_HID)
+#define ORC_W3 (ORC_B2 + ORC_MLP_HID)
+#define ORC_B3 (ORC_W3 + ORC_MLP_HID * ORC_MLP_HID)
+#define ORC_W4 (ORC_B3 + ORC_MLP_HID)
+#define ORC_B4 (ORC_W4 + ORC_MLP_HID * ORC_MLP_OUT)
+#define ORC_NPARAM (ORC_B4 + ORC_MLP_OUT)
+
+static const uint16_t* orc_model_w = NULL;
+
+/* Model used by end mode 2 (BF16 parameters; NULL: mirror everywhere). */
+void orc_set_model(const uint16_t* w) { orc_model_w = w; }
+int orc_mlp_nparam(void) { return ORC_NPARAM; }
+
+float orc_bf16_to_f32(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+uint16_t orc_f32_to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+float orc_mlp_exp(float x) {
+  x = fminf(fmaxf(x, -87.0f), 88.0f);
+  const float k = rintf(x * 1.44269504088896341f);
+  float r = fmaf(-k, 0.693145751953125f, x);
+  r = fmaf(-k, 1.42860682030941723e-6f, r);
+  float p = 1.0f / 5040.0f;
+  p = fmaf(p, r, 1.0f / 720.0f);
+  p = fmaf(p, r, 1.0f / 120.0f);
+  p = fmaf(p, r, 1.0f / 24.0f);
+  p = fmaf(p, r, 1.0f / 6.0f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  return ldexpf(p, (int)k);
+}
+
+float orc_mlp_gelu(float x) {
+  const float x3 = (x * x) * x;
+  const float inner = fmaf(0.044715f, x3, x);
+  float u = 0.7978845608028654f * inner;
+  u = fminf(fmaxf(u, -15.0f), 15.0f);
+  const float e = orc_mlp_exp(u + u);
+  const float t = 1.0f - 2.0f / (1.0f + e);
+  return (0.5f * x) * (1.0f + t);
+}
+
+/* bias + sum w[i*ld] in[i]: fma chains lo (i < n/2, from the bias) and hi
+ * (i >= n/2, from 0) in index order, then lo + hi. */
+static float orc_neuron(const uint16_t* w, int ld, float bias, const float* in, int n) {
+  float lo = bias, hi = 0.0f;
+  for (int i = 0; i < n / 2; i++) lo = fmaf(orc_bf16_to_f32(w[(int64_t)i * ld]), in[i], lo);
+  for (int i = n / 2; i < n; i++) hi = fmaf(orc_bf16_to_f32(w[(int64_t)i * ld]), in[i], hi);
+  return lo + hi;
+}
+
+void orc_mlp_infer(const uint16_t* w, const float* x, float* y) {
+  float h0[ORC_MLP_HID], h1[ORC_MLP_HID];
+  for (int j = 0; j < ORC_MLP_HID; j++)
+    h0[j] = orc_mlp_gelu(orc_neuron(w + ORC_W1 + j, ORC_MLP_HID, orc_bf16_to_f32(w[ORC_B1 + j]), x,
+                                    ORC_MLP_IN));
+  for (int j = 0; j < ORC_MLP_HID; j++)
+    h1[j] = orc_mlp_gelu(orc_neuron(w + ORC_W2 + j, ORC_MLP_HID, orc_bf16_to_f32(w[ORC_B2 + j]),
+                                    h0, ORC_MLP_HID));
+  for (int j = 0; j < ORC_MLP_HID; j++)
+    h0[j] = orc_mlp_gelu(orc_neuron(w + ORC_W3 + j, ORC_MLP_HID, orc_bf16_to_f32(w[ORC_B3 + j]),
+                                    h1, ORC_MLP_HID));
+  for (int j = 0; j < ORC_MLP_OUT; j++)
+    y[j] = orc_neuron(w + ORC_W4 + j, ORC_MLP_OUT, orc_bf16_to_f32(w[ORC_B4 + j]), h0, ORC_MLP_HID);
+}
+
+/* In-place sum of 32 floats by the xor-butterfly tree: for partner offsets
+ * 16, 8, 4, 2, 1, every slot s becomes v[s] + v[s ^ k]; v[0] ends as the sum. */
+static void orc_butterfly32(float* v) {
+  for (int k = 16; k >= 1; k >>= 1) {
+    float nv[32];
+    for (int s = 0; s < 32; s++) nv[s] = v[s] + v[s ^ k];
+    memcpy(v, nv, sizeof(nv));
+  }
+}
+
+/* Model input of one end (0: left, 1: right) from the knots of both
+ * envelopes (time order), and the scales the output is decoded with. */
+void orc_learned_features(const double* Tx, const double* Yx, int64_t px, const double* Tn,
+                          const double* Yn, int64_t pn, int64_t N, int end, float* x,
+                          double* ls_out, float* scale) {
+  const double* Ts[2] = {Tx, Tn};
+  const double* Ys[2] = {Yx, Yn};
+  const int64_t ps[2] = {px, pn};
+  double ls[2][ORC_CTX], ly[2][ORC_CTX];
+  for (int ev = 0; ev < 2; ev++)
+    for (int i = 0; i < ORC_CTX; i++) {
+      const int64_t g = end ? ps[ev] - 1 - i : i;
+      const double t = Ts[ev][g];
+      ls[ev][i] = end ? (double)(N - 1) - t : t;
+      ly[ev][i] = Ys[ev][g];
+    }
+  /* sums over the 32 [env][i] slots: xor-butterfly tree, partner offsets
+   * 16, 8, 4, 2, 1 (slot s = env * 16 + i) */
+  float gv[32], yv[32], vv[32], gap[32];
+  for (int ev = 0; ev < 2; ev++)
+    for (int i = 0; i < ORC_CTX; i++) {
+      gap[ev * 16 + i] = (float)(i == 0 ? ls[ev][0] : ls[ev][i] - ls[ev][i - 1]);
+      gv[ev * 16 + i] = (i == 0) ? 0.0f : gap[ev * 16 + i];
+      yv[ev * 16 + i] = (float)ly[ev][i];
+    }
+  orc_butterfly32(gv);
+  orc_butterfly32(yv);
+  const float G = gv[0] / (float)(2 * (ORC_CTX - 1));
+  const float mu = yv[0] / (float)(2 * ORC_CTX);
+  for (int s = 0; s < 32; s++) {
+    const float d = (float)ly[s >> 4][s & 15] - mu;
+    vv[s] = d * d;
+  }
+  orc_butterfly32(vv);
+  const float sd = fmaxf(sqrtf(vv[0] / (float)(2 * ORC_CTX)), 1e-30f);
+  for (int ev = 0; ev < 2; ev++)
+    for (int i = 0; i < ORC_CTX; i++) {
+      x[ev * 32 + i] = gap[ev * 16 + i] / G - 1.0f;
+      x[ev * 32 + 16 + i] = ((float)ly[ev][i] - mu) / sd;
+    }
+  if (ls_out) {
+    ls_out[0] = ls[0][0];
+    ls_out[1] = ls[1][0];
+  }
+  scale[0] = G;
+  scale[1] = mu;
+  scale[2] = sd;
+}
+
+/* Virtual extrema of both envelopes at both ends: vt / vy [env][end][k],
+ * k = 0 nearest to the signal.  Gaps: first from the end sample,
+ * max(G exp(o) - 1, 0), kept >= 2 from the nearest extremum; then
+ * max(G exp(o), 2) apart (o clamped to [-4, 4]); values mu + sd o. */
+void orc_learned_virtual(const double* Tx, const double* Yx, int64_t px, const double* Tn,
+                         const double* Yn, int64_t pn, int64_t N, const uint16_t* w,
+                         double* vt, double* vy) {
+  for (int end = 0; end < 2; end++) {
+    float x[ORC_MLP_IN], o[ORC_MLP_OUT], sc[3];
+    double ls0[2];
+    orc_learned_features(Tx, Yx, px, Tn, Yn, pn, N, end, x, ls0, sc);
+    orc_mlp_infer(w, x, o);
+    const float G = sc[0], mu = sc[1], sd = sc[2];
+    for (int ev = 0; ev < 2; ev++) {
+      const float* oe = o + ev * 8;
+      const double dir = end ? 1.0 : -1.0;
+      const double e_end = end ? (double)(N - 1) : 0.0;
+      const double t_near = e_end - dir * ls0[ev];
+      const float d0 = fmaxf(G * orc_mlp_exp(fminf(fmaxf(oe[0], -4.0f), 4.0f)) - 1.0f, 0.0f);
+      double q = e_end + dir * (double)d0;
+      if (dir * (q - t_near) < 2.0) q = t_near + dir * 2.0;
+      double* pt = vt + (ev * 2 + end) * ORC_PRED;
+      double* py = vy + (ev * 2 + end) * ORC_PRED;
+      pt[0] = q;
+      for (int k = 1; k < ORC_PRED; k++) {
+        const float gk = fmaxf(G * orc_mlp_exp(fminf(fmaxf(oe[k], -4.0f), 4.0f)), 2.0f);
+        q = q + dir * (double)gk;
+        pt[k] = q;
+      }
+      for (int k = 0; k < ORC_PRED; k++) py[k] = (double)(mu + sd * oe[4 + k]);
+    }
+  }
+}
+
+/* Both envelopes of h with learned ends (mirror when an envelope has fewer
+ * than ORC_CTX extrema or no model is set). */
+static void orc_envelopes_learned(const double* h, int64_t N, int64_t B, int64_t S, double* eu,
+                                  double* el, orc_work* w, uint64_t* hmx, uint64_t* hmn) {
+  const int64_t px = orc_extrema(h, N, 1, w->k);
+  if (hmx) *hmx = orc_hash_idx(w->k, px);
+  for (int64_t j = 0; j < px; j++) { w->Tx[j] = (double)w->k[j]; w->Yx[j] = h[w->k[j]]; }
+  const int64_t pn = orc_extrema(h, N, 0, w->k);
+  if (hmn) *hmn = orc_hash_idx(w->k, pn);
+  for (int64_t j = 0; j < pn; j++) { w->Tn[j] = (double)w->k[j]; w->Yn[j] = h[w->k[j]]; }
+  if (!orc_model_w || px < ORC_CTX || pn < ORC_CTX) {
+    orc_envelope_w(h, N, 1, ORC_END_MIRROR, B, S, eu, w, NULL);
+    orc_envelope_w(h, N, 0, ORC_END_MIRROR, B, S, el, w, NULL);
+    return;
+  }
+  double vt[2 * 2 * ORC_PRED], vy[2 * 2 * ORC_PRED];
+  orc_learned_virtual(w->Tx, w->Yx, px, w->Tn, w->Yn, pn, N, orc_model_w, vt, vy);
+  for (int ev = 0; ev < 2; ev++) {
+    const double* Tk = ev ? w->Tn : w->Tx;
+    const double* Yk = ev ? w->Yn : w->Yx;
+    const int64_t p = ev ? pn : px;
+    const double* lt = vt + (ev * 2 + 0) * ORC_PRED;
+    const double* ly = vy + (ev * 2 + 0) * ORC_PRED;
+    const double* rt = vt + (ev * 2 + 1) * ORC_PRED;
+    const double* ry = vy + (ev * 2 + 1) * ORC_PRED;
+    for (int k = 0; k < ORC_PRED; k++) {
+      w->T[k] = lt[ORC_PRED - 1 - k];
+      w->Y[k] = ly[ORC_PRED - 1 - k];
+    }
+    for (int64_t j = 0; j < p; j++) {
+      w->T[ORC_PRED + j] = Tk[j];
+      w->Y[ORC_PRED + j] = Yk[j];
+    }
+    for (int k = 0; k < ORC_PRED; k++) {
+      w->T[ORC_PRED + p + k] = rt[k];
+      w->Y[ORC_PRED + p + k] = ry[k];
+    }
+    orc_spline_from_knots(w, p + 2 * ORC_PRED, N, B, S, ev ? el : eu);
+  }
+}
+
+/* Array forms for tests. */
+void orc_mlp_exp_n(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; i++) y[i] = orc_mlp_exp(x[i]);
+}
+void orc_mlp_gelu_n(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; i++) y[i] = orc_mlp_gelu(x[i]);
+}
+void orc_mlp_infer_n(const uint16_t* w, const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; i++) orc_mlp_infer(w, x + i * ORC_MLP_IN, y + i * ORC_MLP_OUT);
+}
+void orc_f32_to_bf16_n(const float* x, uint16_t* y, int64_t n) {
+  for (int64_t i = 0; i < n; i++) y[i] = orc_f32_to_bf16(x[i]);
+}
+
+/* Public: both envelopes with learned ends (tests). */
+int orc_envelopes_learned_pub(const double* h, int64_t N, int64_t B, int64_t S, double* eu,
+                              double* el) {
+  orc_work w;
+  if (orc_work_alloc(&w, N) != 0) return -1;
+  orc_envelopes_learned(h, N, B, S, eu, el, &w, NULL, NULL);
+  orc_work_free(&w);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -353,8 +626,12 @@ int orc_emd_1d(const double* x, int64_t N, double tau, int max_sifts, int max_im
       int64_t cmin = orc_extrema(h, N, 0, NULL);
       if (cmax < 1 || cmin < 1) break;
       uint64_t hmx = 0, hmn = 0;
-      orc_envelope_w(h, N, 1, end, B, S, eu, &w, &hmx);
-      orc_envelope_w(h, N, 0, end, B, S, el, &w, &hmn);
+      if (end == ORC_END_LEARNED) {
+        orc_envelopes_learned(h, N, B, S, eu, el, &w, &hmx, &hmn);
+      } else {
+        orc_envelope_w(h, N, 1, end, B, S, eu, &w, &hmx);
+        orc_envelope_w(h, N, 0, end, B, S, el, &w, &hmn);
+      }
       if (trace) {
         uint64_t* tr = trace + ((size_t)n_modes * (size_t)max_sifts + (size_t)s) * 4;
         tr[0] = (uint64_t)cmax; tr[1] = (uint64_t)cmin; tr[2] = hmx; tr[3] = hmn;

@@ -16,13 +16,16 @@ _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "_build" / "liborc.so"
 _lib = None
 
-END = {"natural": 0, "mirror": 1}
+END = {"natural": 0, "mirror": 1, "learned": 2}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(dtype=np.uint16, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_model_keep = None
 
 
 def build(force: bool = False) -> Path:
@@ -65,6 +68,24 @@ def lib():
         L.orc_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_int64, _dp]
         L.orc_mix64.restype = C.c_uint64
         L.orc_mix64.argtypes = [C.c_uint64]
+        L.orc_set_model.restype = None
+        L.orc_set_model.argtypes = [C.c_void_p]
+        L.orc_mlp_nparam.restype = C.c_int
+        for nm, a in (("orc_mlp_exp_n", _fp), ("orc_mlp_gelu_n", _fp)):
+            getattr(L, nm).restype = None
+            getattr(L, nm).argtypes = [a, _fp, C.c_int64]
+        L.orc_mlp_infer_n.restype = None
+        L.orc_mlp_infer_n.argtypes = [_u16p, _fp, _fp, C.c_int64]
+        L.orc_f32_to_bf16_n.restype = None
+        L.orc_f32_to_bf16_n.argtypes = [_fp, _u16p, C.c_int64]
+        L.orc_learned_features.restype = None
+        L.orc_learned_features.argtypes = [_dp, _dp, C.c_int64, _dp, _dp, C.c_int64, C.c_int64,
+                                           C.c_int, _fp, _dp, _fp]
+        L.orc_learned_virtual.restype = None
+        L.orc_learned_virtual.argtypes = [_dp, _dp, C.c_int64, _dp, _dp, C.c_int64, C.c_int64,
+                                          _u16p, _dp, _dp]
+        L.orc_envelopes_learned_pub.restype = C.c_int
+        L.orc_envelopes_learned_pub.argtypes = [_dp, C.c_int64, C.c_int64, C.c_int64, _dp, _dp]
         _lib = L
     return _lib
 
@@ -207,3 +228,80 @@ def hash_indices(k) -> int:
     for v in np.asarray(k, dtype=np.int64):
         s = (s + mix64(int(v))) & 0xFFFFFFFFFFFFFFFF
     return s
+
+
+# ---------------------------------------------------------------------------
+# Learned end extension (the engine's end mode "learned"; no reference
+# counterpart): exact inference arithmetic and virtual-extrema rule.
+# ---------------------------------------------------------------------------
+def bf16(params) -> np.ndarray:
+    """FP32 parameters rounded to BF16 (round to nearest even), as uint16."""
+    a = np.ascontiguousarray(params, dtype=np.float32)
+    out = np.zeros(a.size, dtype=np.uint16)
+    lib().orc_f32_to_bf16_n(a.ravel(), out, a.size)
+    return out
+
+
+def set_model(params):
+    """Model used by end='learned' (FP32 parameters; None: mirror only)."""
+    global _model_keep
+    if params is None:
+        _model_keep = None
+        lib().orc_set_model(None)
+        return
+    _model_keep = bf16(params)
+    if _model_keep.size != lib().orc_mlp_nparam():
+        raise ValueError("wrong parameter count")
+    lib().orc_set_model(_model_keep.ctypes.data)
+
+
+def mlp_exp(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x)
+    lib().orc_mlp_exp_n(x.ravel(), y.ravel(), x.size)
+    return y
+
+
+def mlp_gelu(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x)
+    lib().orc_mlp_gelu_n(x.ravel(), y.ravel(), x.size)
+    return y
+
+
+def mlp_infer(params, x) -> np.ndarray:
+    """Exact FP32 inference over BF16 parameters: x [n, 64] -> [n, 16]."""
+    w = bf16(params)
+    x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 64)
+    y = np.zeros((x.shape[0], 16), dtype=np.float32)
+    lib().orc_mlp_infer_n(w, x, y, x.shape[0])
+    return y
+
+
+def learned_features(tx, yx, tn, yn, N, end):
+    """Model input of one end (0 left, 1 right) and its (G, mu, sd) scales."""
+    tx, yx, tn, yn = map(_f64, (tx, yx, tn, yn))
+    x = np.zeros(64, dtype=np.float32)
+    ls0 = np.zeros(2)
+    sc = np.zeros(3, dtype=np.float32)
+    lib().orc_learned_features(tx, yx, len(tx), tn, yn, len(tn), int(N), int(end), x, ls0, sc)
+    return x, sc
+
+
+def learned_virtual(params, tx, yx, tn, yn, N):
+    """Virtual extrema [env][end][k] positions and values (k=0 nearest)."""
+    tx, yx, tn, yn = map(_f64, (tx, yx, tn, yn))
+    vt = np.zeros(16)
+    vy = np.zeros(16)
+    lib().orc_learned_virtual(tx, yx, len(tx), tn, yn, len(tn), int(N), bf16(params), vt, vy)
+    return vt.reshape(2, 2, 4), vy.reshape(2, 2, 4)
+
+
+def envelopes_learned(h, B=0, S=0):
+    """Upper and lower envelopes with learned ends (model from set_model)."""
+    h = _f64(h)
+    eu = np.zeros(len(h))
+    el = np.zeros(len(h))
+    if lib().orc_envelopes_learned_pub(h, len(h), int(B), int(S), eu, el) != 0:
+        raise MemoryError
+    return eu, el

@@ -1,0 +1,143 @@
+"""GPU: the learned end extension and its training kernels.
+
+* Inference: eemd_mlp_infer is bit-identical to the oracle's exact FP32
+  inference.
+* Training kernels (BF16 MFMA GEMMs, FP32 accumulation): the gradient agrees
+  with the float64 numpy reference. That reference rounds the same operands
+  to BF16 and is itself pinned by finite differences in the CPU tests. A
+  short training run reduces the loss.
+* In-kernel inference: FP64 EEMD with end='learned' reproduces the oracle,
+  with bit-identical extrema traces and IMFs to rounding.
+"""
+import numpy as np
+import pytest
+
+from oracle import orc
+
+import mlp_ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    from eemd4cuda_amd import _lib
+    return _lib.engine()
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.device("cuda", 0))
+
+
+def test_mlp_infer_bit_exact_with_oracle(eng):
+    import torch
+    from eemd4cuda_amd import learned
+    rng = np.random.default_rng(0)
+    p = mlp_ref.random_params(rng)
+    x = rng.standard_normal((96, 64)).astype(np.float32) * 2
+    print("case: mlp inference 96 samples", flush=True)
+    net = learned.MLP(seed=1)
+    net.set_params(p)
+    xd = _dev(x)
+    yd = torch.zeros((96, 16), dtype=torch.float32, device=xd.device)
+    net.infer(xd.data_ptr(), 96, yd.data_ptr())
+    y = yd.cpu().numpy()
+    assert np.array_equal(y, orc.mlp_infer(p, x))
+
+
+def test_mlp_gradient_matches_reference(eng):
+    from eemd4cuda_amd import learned
+    rng = np.random.default_rng(1)
+    p = mlp_ref.random_params(rng)
+    x = rng.standard_normal((64, 64)).astype(np.float32)
+    y = rng.standard_normal((64, 16)).astype(np.float32)
+    print("case: mlp gradient batch 64", flush=True)
+    net = learned.MLP(seed=2)
+    net.set_params(p)
+    xd, yd = _dev(x), _dev(y)
+    g, loss = net.grad(xd.data_ptr(), yd.data_ptr(), 64)
+    l_ref, g_ref = mlp_ref.loss_and_grad(p, x, y, bf16=True)
+    assert abs(loss - l_ref) <= 1e-4 * l_ref
+    offs, _ = mlp_ref.offsets()
+    for l, ((ow, ob), (i, j)) in enumerate(zip(offs, mlp_ref.SIZES)):
+        gw, rw = g[ow:ow + i * j], g_ref[ow:ow + i * j]
+        gb, rb = g[ob:ob + j], g_ref[ob:ob + j]
+        ew = np.linalg.norm(gw - rw) / np.linalg.norm(rw)
+        eb = np.linalg.norm(gb - rb) / np.linalg.norm(rb)
+        print("layer", l, "rel err W %.2e b %.2e" % (ew, eb))
+        assert ew <= 1e-2 and eb <= 1e-2, (l, ew, eb)
+    # the parameters did not move
+    assert np.array_equal(net.params(), p)
+
+
+def test_mlp_training_reduces_loss(eng):
+    from eemd4cuda_amd import learned
+    rng = np.random.default_rng(2)
+    teacher = mlp_ref.random_params(rng)
+    x = rng.standard_normal((2048, 64)).astype(np.float32)
+    y = mlp_ref.forward(teacher, x)[0][3].astype(np.float32)
+    xd, yd = _dev(x), _dev(y)
+    net = learned.MLP(seed=3)
+    print("case: mlp training 300 steps", flush=True)
+    first = None
+    losses = []
+    for s in range(300):
+        b = (s % 16) * 128
+        losses.append(net.step(xd[b:].data_ptr(), yd[b:].data_ptr(), 128, 1e-3))
+        first = first if first is not None else losses[-1]
+    last = float(np.mean(losses[-16:]))
+    print("loss %.4f -> %.4f" % (first, last))
+    assert np.all(np.isfinite(losses))
+    assert last < 0.3 * first
+
+
+@pytest.mark.parametrize("B,S", [(0, 30), (256, 30)])
+def test_learned_ends_fp64_match_oracle(eng, B, S):
+    from eemd4cuda_amd import _lib, learned
+    rng = np.random.default_rng(4)
+    N = 6000
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 41.0) + 0.5 * np.sin(2 * np.pi * t / 9.7) + 0.2 * rng.standard_normal(N)
+    z = rng.standard_normal((3, N))
+    sigma = 0.2 * np.std(x)
+    p = mlp_ref.random_params(rng, scale=0.3)
+    print("case: learned ends fp64 N=6000 E=3 B=%d" % B, flush=True)
+    learned.install(p, eng)
+    prm = _lib.default_params()
+    prm.fp64 = 1
+    prm.end_mode = _lib.END_MODES["learned"]
+    prm.max_imf = 6
+    prm.block = B
+    prm.halo = S
+    prm.members_per_task = 1
+    out = eng.run_batch(x, np.array([0, N]), 3, np.array([sigma]), prm, noise=z,
+                        member_imfs=True, trace=True)
+    orc.set_model(p)
+    try:
+        o = orc.eemd(x, z, sigma, tau=0.0, max_sifts=10, max_imf=6, end="learned", B=B, S=S,
+                     member_imfs=True, trace=True)
+    finally:
+        orc.set_model(None)
+    assert np.array_equal(out["nmode"][0], o["nmode"])
+    assert np.array_equal(out["trace"][0], o["trace"])
+    mimf = out["member_imfs"].reshape(3, 6, N)
+    d = np.max(np.abs(mimf - o["member_imfs"]))
+    print("member IMF max|d| vs oracle:", d)
+    assert d <= 1e-12
+    # the learned ends changed the decomposition against mirror
+    prm.end_mode = _lib.END_MODES["mirror"]
+    mir = eng.run_batch(x, np.array([0, N]), 3, np.array([sigma]), prm, noise=z, member_imfs=True)
+    assert not np.array_equal(mir["member_imfs"], out["member_imfs"])
+
+
+def test_learned_without_model_is_rejected():
+    from eemd4cuda_amd import _lib
+    e2 = _lib.Engine(0)
+    try:
+        prm = _lib.default_params()
+        prm.end_mode = _lib.END_MODES["learned"]
+        with pytest.raises(ValueError, match="model"):
+            e2.run_batch(np.sin(np.arange(500) * 0.1), np.array([0, 500]), 1, np.array([0.0]), prm)
+    finally:
+        e2.close()
