@@ -19,13 +19,14 @@ FLAG_MEMBER_IMFS = 1
 FLAG_TRACE = 2
 FLAG_PHASES = 4
 FLAG_GRAPH = 8
+FLAG_LEVEL_SUMS = 16
 
 class Params(C.Structure):
     _fields_ = [("fp64", C.c_int32), ("end_mode", C.c_int32), ("max_imf", C.c_int32),
                 ("max_sifts", C.c_int32), ("min_extrema", C.c_int32), ("block", C.c_int32),
                 ("halo", C.c_int32), ("threads", C.c_int32), ("members_per_task", C.c_int32),
                 ("flags", C.c_int32), ("tau", C.c_double), ("seed", C.c_uint64),
-                ("wave_windows", C.c_int32), ("reserved", C.c_int32)]
+                ("wave_windows", C.c_int32), ("member0", C.c_int32)]
 
 
 class EEMDBackendError(RuntimeError):
@@ -71,9 +72,10 @@ def load(path: Path | None = None):
                                    C.POINTER(i32), C.POINTER(i32)]
     L.eemd_last_phases.argtypes = [vp, C.POINTER(dbl)]
     L.eemd_graph_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    L.eemd_mem_info.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     for name in ("eemd_ctx_create", "eemd_ctx_destroy", "eemd_run_batch", "eemd_run_batch_dev",
                  "eemd_noise", "eemd_last_timing", "eemd_last_phases", "eemd_autotune",
-                 "eemd_graph_stats"):
+                 "eemd_graph_stats", "eemd_mem_info"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -122,7 +124,8 @@ class Engine:
 
     def run_batch(self, x: np.ndarray, offsets: np.ndarray, E: int, sigmas: np.ndarray,
                   prm: Params, noise: np.ndarray | None = None, member_imfs: bool = False,
-                  trace: bool = False, per_mode: bool = True, graph: bool = False):
+                  trace: bool = False, per_mode: bool = True, graph: bool = False,
+                  level_sums: bool = False):
         """Host-buffer EEMD of the concatenated signals (eemd_run_batch)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
@@ -145,7 +148,7 @@ class Engine:
         nmode = np.zeros(nsig * E, np.int32)
         nsift = np.zeros(nsig * E, np.int32)
         pm = np.zeros(nsig * E * mi, np.int32) if per_mode else None
-        flags = FLAG_GRAPH if graph else 0
+        flags = (FLAG_GRAPH if graph else 0) | (FLAG_LEVEL_SUMS if level_sums else 0)
         mim = None
         if member_imfs:
             flags |= FLAG_MEMBER_IMFS
@@ -213,6 +216,12 @@ class Engine:
                                  int(member0), int(E), int(n), _ptr(out))
         self._check(rc, "eemd_noise")
         return out
+
+    def mem_info(self) -> tuple:
+        """(free, total) device memory in bytes."""
+        a, b = C.c_uint64(), C.c_uint64()
+        self._check(self.lib.eemd_mem_info(self.h, C.byref(a), C.byref(b)), "eemd_mem_info")
+        return a.value, b.value
 
     def graph_stats(self) -> tuple:
         """(graph captures, graph launches) of this context."""

@@ -24,7 +24,7 @@ hipError_t sift_occupancy(int fp64, int nt, int end_mode, int* blocks_per_cu);
 hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start,
                        const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
                        int max_imf, const double* x, double* avg, double* resid,
-                       int32_t* nmodes, int64_t total_n, hipStream_t st);
+                       int32_t* nmodes, int64_t total_n, int sums, hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t launch_fill_f64(double* p, int64_t n, double v, hipStream_t st);
 hipError_t launch_fill_bytes(void* p, int64_t n, hipStream_t st);
@@ -196,6 +196,8 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
   if (p->threads != 0 && p->threads != 256 && p->threads != 512 && p->threads != 1024)
     return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 256, 512 or 1024");
   if (p->members_per_task < 0) return fail(c, EEMD_ERR_BAD_ARG, "members_per_task must be >= 0");
+  if (p->member0 < 0 || (int64_t)p->member0 + E > (int64_t(1) << 31))
+    return fail(c, EEMD_ERR_BAD_ARG, "member0 must be >= 0 and member0 + E < 2^31");
   if (offsets[0] != 0) return fail(c, EEMD_ERR_BAD_ARG, "offsets[0] must be 0");
   for (int s = 0; s < nsig; ++s) {
     const int64_t n = offsets[s + 1] - offsets[s];
@@ -367,7 +369,8 @@ hipError_t enqueue_work(eemd_ctx* c, const Plan& P, SiftArgs A, const eemd_param
   if ((e = launch_mean((const SignalDesc*)D.sigd.p, (int)P.sig.size(),
                        (const int64_t*)D.sig_start.p, (const TaskDesc*)D.taskd.p,
                        (const double*)c->acc.p, (const int32_t*)c->acc_cnt.p, p->max_imf, x, avg,
-                       resid, n_modes, P.total_n, st)) != hipSuccess)
+                       resid, n_modes, P.total_n, (p->flags & EEMD_FLAG_LEVEL_SUMS) ? 1 : 0,
+                       st)) != hipSuccess)
     return e;
   if (events && (e = hipEventRecord(c->ev[2], st)) != hipSuccess) return e;
   return hipSuccess;
@@ -428,6 +431,7 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.nslots = P.grid;
   A.phase = nullptr;
   A.wave_windows = P.wave_windows;
+  A.member0 = p->member0;
   if (p->flags & EEMD_FLAG_PHASES) {
     HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
     A.phase = (uint64_t*)c->phase.p;
@@ -810,6 +814,16 @@ int eemd_set_boundary_model(eemd_ctx* c, const float* params) {
   HIPCHK(c, hipMemcpy(c->mlpw.p, w.data(), sizeof(uint16_t) * EEMD_MLP_NPARAM,
                       hipMemcpyHostToDevice), "model upload");
   c->have_model = true;
+  return EEMD_OK;
+}
+
+int eemd_mem_info(eemd_ctx* c, uint64_t* free_bytes, uint64_t* total_bytes) {
+  if (!c) return EEMD_ERR_BAD_ARG;
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  size_t f = 0, t = 0;
+  HIPCHK(c, hipMemGetInfo(&f, &t), "memory info");
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
   return EEMD_OK;
 }
 

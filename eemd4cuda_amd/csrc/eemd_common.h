@@ -118,7 +118,7 @@ struct SiftArgs {
   uint64_t* phase;          // optional per-workgroup phase clocks [nslots][4]
   int32_t wave_windows;     // FP32: wave-parallel window solve when windows <= this
   int32_t nslots;           // number of workgroups (scratch slots)
-  int32_t pad;
+  int32_t member0;          // Philox member index of the first member
 };
 
 }  // namespace eemd

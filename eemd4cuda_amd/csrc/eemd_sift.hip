@@ -1512,7 +1512,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
       io.sigma = sd.sigma;
       io.seed = A.seed;
       io.sig = (uint32_t)td.signal;
-      io.mem = (uint32_t)mem;
+      io.mem = (uint32_t)(A.member0 + mem);
       io.trace = trace;
       __syncthreads();
       if (tid == 0) {
@@ -1647,10 +1647,12 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
 
 // Level-wise ensemble mean and residual (emd_ref.py:209-217): thread per
 // sample; task accumulators of a signal are added in task (= member) order.
+// With `sums` the levels hold the sums (not divided by the member counts), for
+// callers that combine several batches of one ensemble.
 __global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_start,
                             const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
                             int max_imf, const double* x, double* avg, double* resid,
-                            int32_t* nmodes, int64_t total_n) {
+                            int32_t* nmodes, int64_t total_n, int sums) {
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= total_n) return;
   int lo = 0, hi = nsig - 1;
@@ -1673,7 +1675,7 @@ __global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_
       }
     }
     if (cnt == 0) break;
-    const double a = s / (double)cnt;
+    const double a = sums ? s : s / (double)cnt;
     avg[sd.avg_off + (int64_t)m * sd.n + i] = a;
     rs = (m == 0) ? a : rs + a;
     nm = m + 1;
@@ -1781,12 +1783,12 @@ hipError_t sift_occupancy(int fp64, int nt, int end_mode, int* blocks_per_cu) {
 hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start,
                        const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
                        int max_imf, const double* x, double* avg, double* resid,
-                       int32_t* nmodes, int64_t total_n, hipStream_t st) {
+                       int32_t* nmodes, int64_t total_n, int sums, hipStream_t st) {
   if (total_n <= 0) return hipSuccess;
   const int bs = 256;
   const int64_t g = (total_n + bs - 1) / bs;
   hipLaunchKernelGGL(mean_kernel, dim3((unsigned)g), dim3(bs), 0, st, sig, nsig, sig_start,
-                     task, acc, acc_cnt, max_imf, x, avg, resid, nmodes, total_n);
+                     task, acc, acc_cnt, max_imf, x, avg, resid, nmodes, total_n, sums);
   return hipGetLastError();
 }
 

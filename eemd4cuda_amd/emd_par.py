@@ -18,7 +18,9 @@ its GPU branch _gpu_decompose).  Differences, by design:
 """
 from __future__ import annotations
 
+import os
 import time
+import warnings
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -59,11 +61,33 @@ class EMDResult:
                                 if k not in ("imfs", "residual", "n_modes")})
 
 
+def _auto_batch(eng, N, E, max_imf, explicit_noise):
+    """Members per engine call that fit the free device memory (the
+    reference's rule, emd_par.py:208-222, with this engine's footprint: one
+    FP64 [max_imf, N] accumulator per member plus the noise row)."""
+    free, _ = eng.mem_info()
+    per_member = (max_imf * N + (N if explicit_noise else 0)) * 8
+    fixed = 256 * (N + 1024) * 24          # per-workgroup scratch, generous
+    return max(1, min(E, int((free - fixed) * 0.7 // max(per_member, 1))))
+
+
 def _gpu_decompose(x, E, eps, tau, max_sifts, max_imf, min_extrema, end, seed, batch, fp,
                    noise, B, S, model, members_per_task):
+    """GPU EEMD (emd_par.py:198-254).  Members run in batches that fit the
+    device (batch=None: automatic from the free memory).  Each batch returns
+    its per-level member sums; they are added per level and divided by the
+    per-level member counts, as the reference does for its batches.  With
+    noise='numpy' the batches draw consecutive rows of the reference's
+    default_rng(seed) stream; with 'philox' the counters continue across
+    batches (member0)."""
     N = len(x)
-    if max_imf == -1:
+    capped = max_imf == -1
+    if capped:
         max_imf = 64  # hard cap; the loop stops when extrema run out (emd_par.py:205-206)
+    eng = _lib.engine()
+    if noise not in ("numpy", "philox"):
+        raise ValueError("noise must be 'numpy' or 'philox'")
+    batch = int(batch) if batch else _auto_batch(eng, N, E, max_imf, noise == "numpy")
     prm = _lib.default_params()
     prm.fp64 = 1 if fp == "fp64" else 0
     prm.end_mode = _lib.END_MODES[end]
@@ -76,33 +100,51 @@ def _gpu_decompose(x, E, eps, tau, max_sifts, max_imf, min_extrema, end, seed, b
     prm.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
     prm.members_per_task = int(members_per_task)
     sigma = eps * np.std(x)
-    z = None
-    if noise == "numpy":
-        z = np.random.default_rng(seed).standard_normal((E, N))
-    elif noise != "philox":
-        raise ValueError("noise must be 'numpy' or 'philox'")
-    t0 = time.time()
+    rng = np.random.default_rng(seed) if noise == "numpy" else None
     if end == "learned":
         from . import learned
-        out = learned.run(x, np.array([0, N]), E, np.array([sigma]),
-                          prm, model=model, noise=z)
-    else:
-        out = _lib.engine().run_batch(x, np.array([0, N]), E, np.array([sigma]), prm, noise=z)
+        params = model if model is not None else learned.default_model_path()
+        if isinstance(params, (str, os.PathLike)):
+            params = learned.load(params)
+        learned.install(params, eng)
+    sums = np.zeros((max_imf, N))
+    cnt = np.zeros(max_imf)
+    nsift_all, nmode_all, per_mode_all = [], [], []
+    t0 = time.time()
+    tm = None
+    for b0 in range(0, E, batch):
+        Eb = min(E, b0 + batch) - b0
+        z = rng.standard_normal((Eb, N)) if rng is not None else None
+        prm.member0 = b0
+        out = eng.run_batch(x, np.array([0, N]), Eb, np.array([sigma]), prm, noise=z,
+                            level_sums=True)
+        nm = out["nmode"][0]
+        for m in range(int(nm.max()) if len(nm) else 0):
+            sums[m] += out["avg"].reshape(max_imf, N)[m]
+            cnt[m] += int(np.sum(nm > m))
+        nsift_all.append(out["nsift"][0])
+        nmode_all.append(nm)
+        per_mode_all.append(out["per_mode"][0])
+        tm = out["timing"]
     dt = time.time() - t0
-    n_modes = int(out["n_modes"][0])
-    avg = out["avg"].reshape(max_imf, N)[:n_modes].copy()
-    resid = out["resid"]
-    nsift = out["nsift"][0]
-    tm = out["timing"]
-    stats = dict(method="gpu", E=E, n_modes=n_modes, batch=batch or E,
+    n_modes = int((cnt > 0).sum())
+    avg = sums[:n_modes] / np.where(cnt[:n_modes] > 0, cnt[:n_modes], 1)[:, None]
+    resid = x - avg.sum(axis=0)
+    nsift = np.concatenate(nsift_all)
+    nmode = np.concatenate(nmode_all)
+    reached = bool(capped and np.any(nmode >= max_imf))
+    if reached:
+        warnings.warn("max_imf=-1 is capped at 64 IMFs on the GPU and the cap was reached: the "
+                      "residual may still hold oscillations", RuntimeWarning, stacklevel=3)
+    stats = dict(method="gpu", E=E, n_modes=n_modes, batch=batch,
                  per_trial_total_sifts=nsift.astype(np.int64), gpu_time_s=dt, x=x,
                  total_sifts=int(nsift.sum()), fp_mode=fp, boundary=end,
                  B=int(B), S=int(S), wg=tm["threads"], grid=tm["grid"], tasks=tm["ntasks"],
                  sift_kernel_ms=tm["sift_ms"], mean_kernel_ms=tm["mean_ms"],
                  sample_sifts=int(nsift.sum()) * N, noise=noise,
-                 max_imf_reached=bool(np.any(out["nmode"][0] >= max_imf)))
+                 n_batches=(E + batch - 1) // batch, max_imf_reached=reached)
     if E == 1:
-        stats["per_mode_sifts"] = [int(v) for v in out["per_mode"][0, 0, :n_modes]]
+        stats["per_mode_sifts"] = [int(v) for v in per_mode_all[0][0, :n_modes]]
     return avg, resid, stats
 
 

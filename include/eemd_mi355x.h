@@ -56,6 +56,10 @@ extern "C" {
 #define EEMD_FLAG_PHASES 4      /* per-workgroup phase clocks (eemd_last_phases) */
 #define EEMD_FLAG_GRAPH 8       /* capture the launch sequence in a hipGraph and replay it on
                                    later calls with the same plan, parameters and buffers */
+#define EEMD_FLAG_LEVEL_SUMS 16 /* imf_mean holds per-level sums over the members (not
+                                   divided by the member counts) and residual is x minus
+                                   their sum: for callers that combine batches of one
+                                   ensemble, as emd_par.py:234-250 does */
 
 typedef struct eemd_ctx eemd_ctx;
 
@@ -74,11 +78,11 @@ typedef struct eemd_params {
   uint64_t seed;            /* Philox key when no explicit noise is given */
   int32_t wave_windows;     /* FP32: solve windows a wave at a time when a workgroup has at most
                                this many (0: automatic, < 0: never) */
-  int32_t reserved;
+  int32_t member0;          /* Philox member index of member 0 (batches of one ensemble) */
 } eemd_params;
 
 /* Defaults: FP64, mirror, max_imf 10, max_sifts 10, min_extrema 4, B 1024, S 30,
- * automatic workgroup / task sizes, tau 0, seed 0. */
+ * automatic workgroup / task sizes, tau 0, seed 0, member0 0. */
 void eemd_params_default(eemd_params* p);
 
 int eemd_ctx_create(int device, eemd_ctx** out);
@@ -150,6 +154,10 @@ int eemd_noise(eemd_ctx* ctx, uint64_t seed, int32_t signal, int32_t member0, in
  * sift_ms and 0 in mean_ms. */
 int eemd_last_timing(eemd_ctx* ctx, double* sift_ms, double* mean_ms, int32_t* threads,
                      int32_t* grid, int32_t* ntasks);
+
+/* Free and total device memory (hipMemGetInfo) of the context's device: the
+ * reference sizes its batches from the free memory (emd_par.py:208-222). */
+int eemd_mem_info(eemd_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 
 /* Graph captures and graph launches (EEMD_FLAG_GRAPH calls) since the context
  * was created: a replayed call adds a launch without a capture. */

@@ -305,3 +305,42 @@ def envelopes_learned(h, B=0, S=0):
     if lib().orc_envelopes_learned_pub(h, len(h), int(B), int(S), eu, el) != 0:
         raise MemoryError
     return eu, el
+
+
+def ceemdan(x, W, eps=0.2, tau=0.25, max_sifts=50, end="natural", B=0, S=0):
+    """CEEMDAN as emd_ref.py:220-259 defines it, on the C oracle.
+
+    W: [E, N] standard normals (the reference draws default_rng(seed)).
+    Stage 0 decomposes every W[e] (only the IMF counts are used); each stage
+    k takes the first IMF of resid + eps*std(resid)*W[e] for every e (the raw
+    noise, as the reference does), averages them, and subtracts the mean from
+    resid; stops when resid is ~0 or every noise ran out of IMFs.
+    Returns (imfs [K, N], resid, dict(noise_nimf, stage_sifts [K, E])).
+    """
+    x = _f64(x)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    E, N = W.shape
+    noise_n = [emd_1d(W[e], tau=tau, max_sifts=max_sifts, max_imf=-1, min_extrema=4, end=end,
+                      B=B, S=S)[0].shape[0] for e in range(E)]
+    resid = x.copy()
+    c_imfs, stage_sifts = [], []
+    while True:
+        if np.max(np.abs(resid)) < 1e-12 or np.std(resid) < 1e-10:
+            break
+        stage = len(c_imfs)
+        beta = eps * np.std(resid)
+        c = np.zeros(N)
+        ns = []
+        for e in range(E):
+            imfs, _, st = emd_1d(resid + beta * W[e], tau=tau, max_sifts=max_sifts, max_imf=1,
+                                 min_extrema=4, end=end, B=B, S=S)
+            c += imfs[0]
+            ns.append(st["per_mode_sifts"][0])
+        c /= E
+        c_imfs.append(c)
+        stage_sifts.append(ns)
+        resid = resid - c
+        if all(stage >= n for n in noise_n):
+            break
+    return np.array(c_imfs), resid, dict(noise_nimf=np.array(noise_n, np.int32),
+                                         stage_sifts=np.array(stage_sifts, np.int32))

@@ -2,7 +2,8 @@
 
 Run once in a container that has the reference at /root/reference:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # EEMD / EMD / degenerate sets
+    python tests/golden/make_golden.py ceemdan    # CEEMDAN set only
 
 emd_ref.emd_1d's exact trajectory under numpy arithmetic (numpy's SIMD power
 in the spline cube, pairwise np.sum in the SD test, searchsorted intervals)
@@ -192,5 +193,45 @@ def main():
     print("wrote", sorted(p.name for p in HERE.glob("golden_*")))
 
 
+def golden_ceemdan():
+    """emd_ref.ceemdan (emd_ref.py:220-259), serial, with the per-stage
+    first-IMF sift counts and the noise-EMD IMF counts recorded by wrapping
+    the reference's trial workers (the reference is not modified)."""
+    out = {}
+    x = signals.chirp2048()[:1024]
+    orig_1imf, orig_eemd = R._trial_1imf, R._trial_eemd
+    for end, tau, max_sifts in (("natural", 0.25, 50), ("mirror", 0.2, 20)):
+        stage_ns, noise_n = [], []
+
+        def t1(args):
+            r = orig_1imf(args)
+            stage_ns.append(r[1])
+            return r
+
+        def te(args):
+            r = orig_eemd(args)
+            noise_n.append(len(r[0]))
+            return r
+
+        R._trial_1imf, R._trial_eemd = t1, te
+        try:
+            c, resid, info = R.ceemdan(x, E=4, eps=0.2, parallel=False, seed=0, tau=tau,
+                                       max_sifts=max_sifts, end=end)
+        finally:
+            R._trial_1imf, R._trial_eemd = orig_1imf, orig_eemd
+        k = "%s_" % end
+        out[k + "imfs"] = np.asarray(c)
+        out[k + "resid"] = resid
+        out[k + "stage_sifts"] = np.array(stage_ns, np.int32).reshape(-1, 4)
+        out[k + "noise_nimf"] = np.array(noise_n, np.int32)
+        out[k + "tau"] = tau
+        out[k + "max_sifts"] = max_sifts
+        print("ceemdan %s: %d IMFs, noise IMFs %s" % (end, len(c), noise_n))
+    np.savez_compressed(HERE / "golden_ceemdan.npz", x=x, E=4, eps=0.2, seed=0, **out)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["ceemdan"]:
+        golden_ceemdan()
+        sys.exit(0)
     main()
