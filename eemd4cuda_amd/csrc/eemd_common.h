@@ -51,13 +51,13 @@ struct TaskDesc {
 // Per-workgroup scratch (global memory).  Offsets in bytes, 256-aligned.
 // Knot records are {t, y, M, -} in the working type: 16 bytes in FP32 mode,
 // 32 bytes in FP64 mode.
-constexpr int KREC_LEN = 4;
 constexpr int VSLOTS = 2 * MAX_VIRT;  // virtual-knot records after the segment regions
 
 struct ScratchLayout {
   int64_t h, r;       // working h and residual (T[npad])
   int64_t bits;       // extrema bitmaps uint8[2 sets][2 env][npad/8]
-  int64_t kr;         // knot records [2 sets][2 env][nseg*cap + VSLOTS] x 4 T
+  int64_t kty;        // knot (position, value) pairs [2 sets][2 env][nseg*cap + VSLOTS] x 2 T
+  int64_t km;         // knot second derivatives, same slots, T each
   int64_t wcx;        // window Thomas forward sweep {c', x'} [rows][cols]
                       // (double2 in FP64 mode, float2 in FP32 mode)
   int64_t total;
@@ -72,7 +72,8 @@ __host__ __device__ inline ScratchLayout make_layout(int tbytes, int64_t npad, i
   L.h = o;    o = align256(o + npad * tbytes);
   L.r = o;    o = align256(o + npad * tbytes);
   L.bits = o; o = align256(o + 4 * (npad / 8));
-  L.kr = o;   o = align256(o + 4 * (nseg * cap + VSLOTS) * KREC_LEN * tbytes);
+  L.kty = o;  o = align256(o + 4 * (nseg * cap + VSLOTS) * 2 * tbytes);
+  L.km = o;   o = align256(o + 4 * (nseg * cap + VSLOTS) * tbytes);
   L.wcx = o;  o = align256(o + rows * cols * 2 * tbytes);
   L.total = o;
   return L;

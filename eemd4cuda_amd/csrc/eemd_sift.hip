@@ -17,12 +17,13 @@
 //   bits        1 bit per sample per envelope: bit b <=> sample b-1 is an
 //               extremum (the shift lets a chunk decide its bits from values
 //               it has already produced); two sets (current / next sift)
-//   kr          knot records {t, y, M, -} of 4 T per knot, per (set, envelope),
-//               stored per segment region (the wave that owns a signal segment
-//               writes its knots to its own region, so no barrier is needed
-//               inside a pass); VSLOTS records after the regions hold the
-//               virtual end knots.  M (spline second derivative) is written
-//               into the record by the solve.
+//   kty, km     knots per (set, envelope) by slot: (position, value) pairs and
+//               spline second derivatives M in a separate array, so the
+//               extrema pass writes pairs and the solve writes M densely.
+//               Slots are grouped per segment region (the wave that owns a
+//               signal segment writes its knots to its own region, so no
+//               barrier is needed inside a pass); VSLOTS slots after the
+//               regions hold the virtual end knots.
 //   wcx         forward sweep {c', x'} of the window Thomas solves,
 //               [rows][cols] pairs, column = the lane that owns the window
 //
@@ -193,19 +194,32 @@ struct Mem {
   T* r;
   uint8_t* bits;    // [2 sets][2 env][npad/8]
   int64_t bstride;  // npad/8
-  T* kr;            // knot records [2][2][NW*cap + VSLOTS], KREC_BYTES each
-  int64_t rstride;  // records per (set, env)
+  // knots per (set, env), slot-indexed: positions and values as pairs,
+  // spline second derivatives in their own array (dense writes by the
+  // extrema pass and by the solve)
+  typename std::conditional<std::is_same<T, float>::value, float2, double2>::type* kty;
+  T* km;
+  int64_t rstride;  // slots per (set, env)
   int64_t vbase;    // slot of virtual knot 0 (= NW*cap)
   void* wcx;        // window forward sweep {c', x'} [rows][cols] (T2)
   int64_t rows, cols;
 };
 
 template <typename T>
-constexpr int rec_len() { return KREC_LEN; }
+using Pair = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+
+// Index of slot `slot` of (set, env) in kty / km.
+template <typename T>
+__device__ __forceinline__ int64_t kix(const Mem<T>& M, int set, int env, int64_t slot) {
+  return (int64_t)(set * 2 + env) * M.rstride + slot;
+}
 
 template <typename T>
-__device__ __forceinline__ T* rec(const Mem<T>& M, int set, int env, int64_t slot) {
-  return M.kr + ((int64_t)(set * 2 + env) * M.rstride + slot) * rec_len<T>();
+__device__ __forceinline__ void put_knot(Pair<T>* p, int q, T val) {
+  Pair<T> v;
+  v.x = (T)q;
+  v.y = val;
+  *p = v;
 }
 
 // Slot of real knot g (0-based in time order) of (set, env); w is a segment
@@ -238,19 +252,12 @@ template <typename T, int NW>
 __device__ __forceinline__ Knot<T> knot_at(const Shared<T, NW>& S, const Mem<T>& M, int set,
                                            int env, int e, int& w) {
   const int64_t slot = ext_slot(S, M, set, env, e, w);
-  const T* r = rec(M, set, env, slot);
+  const int64_t ix = kix(M, set, env, slot);
+  const Pair<T> v = M.kty[ix];
   Knot<T> k;
-  if constexpr (std::is_same<T, float>::value) {
-    const float4 v = *reinterpret_cast<const float4*>(r);
-    k.t = v.x;
-    k.y = v.y;
-    k.m = v.z;
-  } else {
-    const double2 v = *reinterpret_cast<const double2*>(r);
-    k.t = v.x;
-    k.y = v.y;
-    k.m = r[2];
-  }
+  k.t = v.x;
+  k.y = v.y;
+  k.m = M.km[ix];
   k.code = (int)slot;
   return k;
 }
@@ -353,18 +360,11 @@ struct EnvChunk {
 };
 
 template <typename T>
-__device__ __forceinline__ void load_tym(const T* r, T& t, T& y, T& m) {
-  if constexpr (std::is_same<T, float>::value) {
-    const float4 v = *reinterpret_cast<const float4*>(r);
-    t = v.x;
-    y = v.y;
-    m = v.z;
-  } else {
-    const double2 v = *reinterpret_cast<const double2*>(r);
-    t = v.x;
-    y = v.y;
-    m = r[2];
-  }
+__device__ __forceinline__ void load_tym(const Mem<T>& M, int64_t ix, T& t, T& y, T& m) {
+  const Pair<T> v = M.kty[ix];
+  t = v.x;
+  y = v.y;
+  m = M.km[ix];
 }
 
 // Stage 2 of the pipeline for one envelope: interval indices of the chunk
@@ -393,7 +393,7 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   const int cl7 = c.base + __popc(byte);
   const int lo = __shfl(cl0, 0);
   const int hi = __shfl(cl7, WAVE - 1);
-  T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
+  const int64_t rb = kix(M, cur, ev, 0);
   if (lo >= 1 && hi <= p - 1) {
     c.fast = 1;
     c.e0 = cl0 + nl - 1;
@@ -405,22 +405,22 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
     const int rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
     const int need = span + 2;  // knots e0 .. e0+span+1, all real
     if (rem >= need) {
-      const T* r = rb + s0 * rec_len<T>();
-      load_tym(r, c.t0, c.y0, c.m0);
-      load_tym(r + rec_len<T>(), c.t1, c.y1, c.m1);
-      if (need > 2) load_tym(r + 2 * rec_len<T>(), c.t2, c.y2, c.m2);
-      if (cls >= 1 && need > 3) load_tym(r + 3 * rec_len<T>(), c.t3, c.y3, c.m3);
-      if (cls >= 2 && need > 4) load_tym(r + 4 * rec_len<T>(), c.t4, c.y4, c.m4);
-      if (cls >= 2 && need > 5) load_tym(r + 5 * rec_len<T>(), c.t5, c.y5, c.m5);
+      const int64_t r = rb + s0;
+      load_tym(M, r, c.t0, c.y0, c.m0);
+      load_tym(M, r + 1, c.t1, c.y1, c.m1);
+      if (need > 2) load_tym(M, r + 2, c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) load_tym(M, r + 3, c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) load_tym(M, r + 4, c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) load_tym(M, r + 5, c.t5, c.y5, c.m5);
     } else {
       int wh = wg;
-      auto at = [&](int k) { return rb + real_slot(S, M, cur, ev, g0 + k, wh) * rec_len<T>(); };
-      load_tym(at(0), c.t0, c.y0, c.m0);
-      load_tym(at(1), c.t1, c.y1, c.m1);
-      if (need > 2) load_tym(at(2), c.t2, c.y2, c.m2);
-      if (cls >= 1 && need > 3) load_tym(at(3), c.t3, c.y3, c.m3);
-      if (cls >= 2 && need > 4) load_tym(at(4), c.t4, c.y4, c.m4);
-      if (cls >= 2 && need > 5) load_tym(at(5), c.t5, c.y5, c.m5);
+      auto at = [&](int k) { return rb + real_slot(S, M, cur, ev, g0 + k, wh); };
+      load_tym(M, at(0), c.t0, c.y0, c.m0);
+      load_tym(M, at(1), c.t1, c.y1, c.m1);
+      if (need > 2) load_tym(M, at(2), c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) load_tym(M, at(3), c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) load_tym(M, at(4), c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) load_tym(M, at(5), c.t5, c.y5, c.m5);
     }
     return cls;
   }
@@ -434,15 +434,15 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   auto at = [&](int k) {
     int e = c.e0 + k;
     e = e > last ? last : e;
-    return rb + ext_slot(S, M, cur, ev, e, wh) * rec_len<T>();
+    return rb + ext_slot(S, M, cur, ev, e, wh);
   };
-  load_tym(at(0), c.t0, c.y0, c.m0);
-  load_tym(at(1), c.t1, c.y1, c.m1);
-  load_tym(at(2), c.t2, c.y2, c.m2);
-  if (cls >= 1) load_tym(at(3), c.t3, c.y3, c.m3);
+  load_tym(M, at(0), c.t0, c.y0, c.m0);
+  load_tym(M, at(1), c.t1, c.y1, c.m1);
+  load_tym(M, at(2), c.t2, c.y2, c.m2);
+  if (cls >= 1) load_tym(M, at(3), c.t3, c.y3, c.m3);
   if (cls >= 2) {
-    load_tym(at(4), c.t4, c.y4, c.m4);
-    load_tym(at(5), c.t5, c.y5, c.m5);
+    load_tym(M, at(4), c.t4, c.y4, c.m4);
+    load_tym(M, at(5), c.t5, c.y5, c.m5);
   }
   return cls;
 }
@@ -513,14 +513,6 @@ __device__ __forceinline__ void env_eval_any(const Shared<T, NW>& S, int ev, int
   }
 }
 
-template <typename T>
-__device__ __forceinline__ void put_knot(T* r, int q, T val) {
-  if constexpr (std::is_same<T, float>::value)
-    *reinterpret_cast<float2*>(r) = make_float2((float)q, val);
-  else
-    *reinterpret_cast<double2*>(r) = make_double2((double)q, val);
-}
-
 // Extrema of the new values v (bit b <=> position b-1), written to the
 // bitmaps of set nxt and compacted, in time order, into the wave's segment
 // region (emd_ref.py:84-87).  p1/p2 come from the lane before (or the carry
@@ -571,7 +563,7 @@ __device__ __forceinline__ void detect_compact(const Mem<T>& M, int nxt, int64_t
         if ((bits >> k) & 1) {
           const int q = (int)(i0 + k - 1);
           const T val = (k == 0) ? p1 : v[k - 1];
-          if (o < kcap) put_knot(rec(M, nxt, ev, seg_slot + o), q, val);
+          if (o < kcap) put_knot<T>(M.kty + kix(M, nxt, ev, seg_slot + o), q, val);
           if (trace) hs[ev] += mix64((unsigned long long)q);
           ++o;
         }
@@ -793,7 +785,7 @@ __device__ __forceinline__ void fixup(Shared<T, NW>& S, const Mem<T>& M, const T
       const bool hit = ev == 0 ? (b > a && b > c) : (b < a && b < c);
       if (!hit) continue;
       const int bg = S.beg[set][ev][w] - 1;
-      put_knot(rec(M, set, ev, (int64_t)w * M.cap + bg), (int)q, b);
+      put_knot<T>(M.kty + kix(M, set, ev, (int64_t)w * M.cap + bg), (int)q, b);
       S.beg[set][ev][w] = bg;
       S.cnt[set][ev][w] += 1;
       uint8_t* bp = M.bits + (int64_t)(set * 2 + ev) * M.bstride + seg_lo / 8;
@@ -843,17 +835,23 @@ __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, in
       const T lt = a.t - (b.t - a.t), rt = d.t + (d.t - c.t);
       S.vt[ev][0] = lt;
       S.vt[ev][MAX_VIRT] = rt;
-      T* vl = rec(M, cur, ev, M.vbase);
-      T* vr = rec(M, cur, ev, M.vbase + MAX_VIRT);
-      vl[0] = lt; vl[1] = b.y; vl[2] = (T)0;
-      vr[0] = rt; vr[1] = c.y; vr[2] = (T)0;
+      const int64_t il = kix(M, cur, ev, M.vbase), ir = kix(M, cur, ev, M.vbase + MAX_VIRT);
+      Pair<T> vl, vr;
+      vl.x = lt;
+      vl.y = b.y;
+      vr.x = rt;
+      vr.y = c.y;
+      M.kty[il] = vl;
+      M.kty[ir] = vr;
+      M.km[il] = (T)0;
+      M.km[ir] = (T)0;
       S.nl[ev] = 1;
       S.nr[ev] = 1;
       S.P[ev] = p + 2;
     } else {
       // natural: M = 0 at the first and last real knot
-      rec(M, cur, ev, a.code)[2] = (T)0;
-      rec(M, cur, ev, d.code)[2] = (T)0;
+      M.km[kix(M, cur, ev, a.code)] = (T)0;
+      M.km[kix(M, cur, ev, d.code)] = (T)0;
     }
   }
 }
@@ -885,10 +883,10 @@ __device__ __forceinline__ void learned_ends(Shared<T, NW>& S, LearnedLDS& L, co
     const int p = S.p[ev];
     int wh = end ? NW - 1 : 0;
     const int64_t slot = real_slot(S, M, cur, ev, end ? p - 1 - i : i, wh);
-    const T* r = rec(M, cur, ev, slot);
-    const double t = (double)r[0];
+    const Pair<T> r = M.kty[kix(M, cur, ev, slot)];
+    const double t = (double)r.x;
     const double ls = end ? (double)(M.n - 1) - t : t;
-    const float yf = (float)(double)r[1];
+    const float yf = (float)(double)r.y;
     const double lsp = __shfl_up(ls, 1);
     const float gap = (float)(i == 0 ? ls : ls - lsp);
     float gs = (i == 0) ? 0.0f : gap, ys = yf;
@@ -974,10 +972,12 @@ __device__ __forceinline__ void learned_ends(Shared<T, NW>& S, LearnedLDS& L, co
       ov = kk == 3 ? o[7] : ov;
       // left end: slots in time order, outermost first; right end: nearest first
       const int slot = end ? MAX_VIRT + kk : MLP_PRED - 1 - kk;
-      T* r = rec(M, cur, ev, M.vbase + slot);
-      r[0] = (T)q;
-      r[1] = (T)(mu + sd * ov);
-      r[2] = (T)0;
+      const int64_t ix = kix(M, cur, ev, M.vbase + slot);
+      Pair<T> r;
+      r.x = (T)q;
+      r.y = (T)(mu + sd * ov);
+      M.kty[ix] = r;
+      M.km[ix] = (T)0;
       S.vt[ev][slot] = (T)q;
       if (kk == 0) {
         if (end) S.nr[ev] = MLP_PRED;
@@ -991,7 +991,7 @@ __device__ __forceinline__ void learned_ends(Shared<T, NW>& S, LearnedLDS& L, co
 template <typename T, int NW>
 __device__ __forceinline__ void store_m(Shared<T, NW>& S, const Mem<T>& M, int cur, int env,
                                         int code, double val) {
-  rec(M, cur, env, code)[2] = (T)val;
+  M.km[kix(M, cur, env, code)] = (T)val;
 }
 
 // Sequential reader of the knot slots of extended indices e, e+1, ... of
@@ -1140,21 +1140,22 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
   const int s = w0 + ln * sb;
   const int e = (s + sb - 1 < w1) ? s + sb - 1 : w1;
   const bool act = ln < nact;
-  T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
+  const Pair<T>* const tyb = M.kty + kix(M, cur, ev, 0);
+  T* const mb = M.km + kix(M, cur, ev, 0);
   float* const scr = reinterpret_cast<float*>(M.wcx) + (int64_t)w * (3 * 64 * WAVE);
   // scratch row j of this lane: scr[(j*3 + comp) * WAVE + ln], sb <= 64
   double Ps = 0, Qs = 0, Rs = 0, Pe = 0, Qe = 0, Re = 0;
   if (act) {
     KStream ks;
     ks_seek(ks, S, M, cur, ev, s);
-    float2 ka = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
-    float2 kb = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    float2 ka = tyb[ks_next(ks, S, M, cur, ev)];
+    float2 kb = tyb[ks_next(ks, S, M, cur, ev)];
     double at = ka.x, ay = ka.y, bt = kb.x, by = kb.y;
     double sl0 = (by - ay) * __builtin_amdgcn_rcp(bt - at);
     double cp = 0.0, dp = 0.0, vp = 0.0;
 #pragma unroll 1
     for (int q = s; q <= e; ++q) {
-      const float2 kc = *reinterpret_cast<const float2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+      const float2 kc = tyb[ks_next(ks, S, M, cur, ev)];
       const double ct = kc.x, cy = kc.y;
       const double H0 = bt - at, H1 = ct - bt;
       const double d = 2.0 * (H0 + H1);
@@ -1238,7 +1239,7 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
     double x = Pe + Qe * alpha + Re * beta;
     {
       const int64_t sl = kr_prev(kr, S, M, cur, ev, nl, p);
-      if (e >= o0 && e <= o1) rb[sl * rec_len<T>() + 2] = (T)x;
+      if (e >= o0 && e <= o1) mb[sl] = (T)x;
     }
 #pragma unroll 1
     for (int j = e - s - 1; j >= 0; --j) {
@@ -1248,7 +1249,7 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
       x = dd + vv * alpha - c * x;
       const int64_t sl = kr_prev(kr, S, M, cur, ev, nl, p);
       const int q = s + j;
-      if (q >= o0 && q <= o1) rb[sl * rec_len<T>() + 2] = (T)x;
+      if (q >= o0 && q <= o1) mb[sl] = (T)x;
     }
   }
 }
@@ -1264,7 +1265,7 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
 template <typename T, int NW>
 __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur, int Bp, int Sp,
                                       int wave_windows) {
-  constexpr int PF = 8;
+  constexpr int PF = 16;
   using T2 = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
   T2* const wcx = reinterpret_cast<T2*>(M.wcx);
   int nwin[2], nrow[2], Be[2];
@@ -1317,12 +1318,13 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     if (w0 < 0) w0 = 0;
     int w1 = o1 + Sp;
     if (w1 > n - 1) w1 = n - 1;
-    T* const rb = M.kr + (int64_t)(cur * 2 + ev) * M.rstride * rec_len<T>();
+    const Pair<T>* const tyb = M.kty + kix(M, cur, ev, 0);
+    T* const mb = M.km + kix(M, cur, ev, 0);
 
     KStream ks;
     ks_seek(ks, S, M, cur, ev, w0);
-    const T2 ka = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
-    const T2 kb = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+    const T2 ka = tyb[ks_next(ks, S, M, cur, ev)];
+    const T2 kb = tyb[ks_next(ks, S, M, cur, ev)];
     double at = (double)ka.x, ay = (double)ka.y, bt = (double)kb.x, by = (double)kb.y;
     double sl0 = qdiv<T>(by - ay, bt - at);
     double cprev = 0.0, xprev = 0.0;
@@ -1335,7 +1337,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
       if (enext <= elast) {
-        nk[u] = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+        nk[u] = tyb[ks_next(ks, S, M, cur, ev)];
         ++enext;
       } else {
         nk[u].x = (T)0;
@@ -1348,7 +1350,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
 #pragma unroll
       for (int u = 0; u < PF; ++u) {
         if (enext <= elast) {
-          fk[u] = *reinterpret_cast<const T2*>(rb + ks_next(ks, S, M, cur, ev) * rec_len<T>());
+          fk[u] = tyb[ks_next(ks, S, M, cur, ev)];
           ++enext;
         } else {
           fk[u].x = (T)0;
@@ -1400,7 +1402,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     double xn = xprev;
     {
       const int64_t s1 = kr_prev(kr, S, M, cur, ev, nl, p);
-      if (w1 >= o0 && w1 <= o1) rb[s1 * rec_len<T>() + 2] = (T)xn;
+      if (w1 >= o0 && w1 <= o1) mb[s1] = (T)xn;
     }
     T2 cx[PF];
 #pragma unroll
@@ -1422,7 +1424,7 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
         if (q >= w0) {
           const double xi = (double)cx[u].y - (double)cx[u].x * xn;
           const int64_t sq = kr_prev(kr, S, M, cur, ev, nl, p);
-          if (q >= o0 && q <= o1) rb[sq * rec_len<T>() + 2] = (T)xi;
+          if (q >= o0 && q <= o1) mb[sq] = (T)xi;
           xn = xi;
         }
       }
@@ -1453,7 +1455,8 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   M.r = reinterpret_cast<T*>(scr + LY.r);
   M.bits = reinterpret_cast<uint8_t*>(scr + LY.bits);
   M.bstride = A.npad / 8;
-  M.kr = reinterpret_cast<T*>(scr + LY.kr);
+  M.kty = reinterpret_cast<Pair<T>*>(scr + LY.kty);
+  M.km = reinterpret_cast<T*>(scr + LY.km);
   M.rstride = (int64_t)NW * A.cap + VSLOTS;
   M.vbase = (int64_t)NW * A.cap;
   M.wcx = reinterpret_cast<void*>(scr + LY.wcx);
