@@ -1344,17 +1344,56 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
         nk[u].y = (T)0;
       }
     }
+    const int nl_ev = S.nl[ev], p_ev = S.p[ev];
 #pragma unroll 1
     for (int q0 = w0; q0 <= w1; q0 += PF) {
       T2 fk[PF];
+      bool wide = false;
+      if constexpr (std::is_same<T, float>::value) {
+        // the next PF knots are consecutive slots of one real run inside the
+        // window: read them 2 pairs per 16-byte load (half the requests of
+        // per-pair loads; each lane streams its own window, so these are
+        // uncoalesced and request-bound)
+        const int g = ks.e - nl_ev;
+        wide = g >= 0 && g + PF < p_ev && ks.rem > PF && enext + PF - 1 <= elast;
+      }
+      if (wide) {
+        const T2* src = tyb + ks.slot;
+        if ((ks.slot & 1) == 0) {
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        if (enext <= elast) {
-          fk[u] = tyb[ks_next(ks, S, M, cur, ev)];
-          ++enext;
+          for (int u = 0; u < PF; u += 2) {
+            const float4 v = *reinterpret_cast<const float4*>(src + u);
+            fk[u].x = v.x;
+            fk[u].y = v.y;
+            fk[u + 1].x = v.z;
+            fk[u + 1].y = v.w;
+          }
         } else {
-          fk[u].x = (T)0;
-          fk[u].y = (T)0;
+          fk[0] = src[0];
+#pragma unroll
+          for (int u = 1; u < PF - 1; u += 2) {
+            const float4 v = *reinterpret_cast<const float4*>(src + u);
+            fk[u].x = v.x;
+            fk[u].y = v.y;
+            fk[u + 1].x = v.z;
+            fk[u + 1].y = v.w;
+          }
+          fk[PF - 1] = src[PF - 1];
+        }
+        ks.slot += PF;
+        ks.rem -= PF;
+        ks.e += PF;
+        enext += PF;
+      } else {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          if (enext <= elast) {
+            fk[u] = tyb[ks_next(ks, S, M, cur, ev)];
+            ++enext;
+          } else {
+            fk[u].x = (T)0;
+            fk[u].y = (T)0;
+          }
         }
       }
 #pragma unroll
@@ -1400,9 +1439,32 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     KRev kr;
     kr_seek(kr, S, M, cur, ev, w1 + 1);
     double xn = xprev;
+    // FP32: owned M values of adjacent slots (s even, s + 1) leave as one
+    // 8-byte store; an odd slot waits one row for its even neighbour
+    int64_t pend_s = -1;
+    T pend_v = (T)0;
+    auto put_m = [&](int64_t sq, T v) {
+      if constexpr (std::is_same<T, float>::value) {
+        if (pend_s == sq + 1 && (sq & 1) == 0) {
+          *reinterpret_cast<float2*>(mb + sq) = make_float2(v, pend_v);
+          pend_s = -1;
+          return;
+        }
+        if (pend_s >= 0) mb[pend_s] = pend_v;
+        pend_s = -1;
+        if (sq & 1) {
+          pend_s = sq;
+          pend_v = v;
+        } else {
+          mb[sq] = v;
+        }
+      } else {
+        mb[sq] = v;
+      }
+    };
     {
       const int64_t s1 = kr_prev(kr, S, M, cur, ev, nl, p);
-      if (w1 >= o0 && w1 <= o1) mb[s1] = (T)xn;
+      if (w1 >= o0 && w1 <= o1) put_m(s1, (T)xn);
     }
     T2 cx[PF];
 #pragma unroll
@@ -1424,13 +1486,14 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
         if (q >= w0) {
           const double xi = (double)cx[u].y - (double)cx[u].x * xn;
           const int64_t sq = kr_prev(kr, S, M, cur, ev, nl, p);
-          if (q >= o0 && q <= o1) mb[sq] = (T)xi;
+          if (q >= o0 && q <= o1) put_m(sq, (T)xi);
           xn = xi;
         }
       }
 #pragma unroll
       for (int u = 0; u < PF; ++u) cx[u] = fx[u];
     }
+    if (pend_s >= 0) mb[pend_s] = pend_v;
   }
 }
 
