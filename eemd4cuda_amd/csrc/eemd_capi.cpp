@@ -196,6 +196,9 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
   if (p->threads != 0 && p->threads != 256 && p->threads != 512 && p->threads != 1024)
     return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 256, 512 or 1024");
   if (p->members_per_task < 0) return fail(c, EEMD_ERR_BAD_ARG, "members_per_task must be >= 0");
+  if (p->end_mode == EEMD_END_LEARNED && p->fp64 && p->threads == 1024)
+    return fail(c, EEMD_ERR_BAD_ARG,
+                "FP64 with learned ends needs threads 256 or 512 (LDS holds the model)");
   if (p->member0 < 0 || (int64_t)p->member0 + E > (int64_t(1) << 31))
     return fail(c, EEMD_ERR_BAD_ARG, "member0 must be >= 0 and member0 + E < 2^31");
   if (offsets[0] != 0) return fail(c, EEMD_ERR_BAD_ARG, "offsets[0] must be 0");

@@ -62,6 +62,10 @@ struct Shared {
   int32_t pm[MAX_IMF_CAP];
   int32_t task;
   double sd;
+  // per-wave staging of one envelope's new knots of a chunk (at most 4 per
+  // lane: strict extrema are >= 2 samples apart), copied out coalesced
+  typename std::conditional<std::is_same<T, float>::value, float2, double2>::type
+      stage[NW][4 * WAVE];
 };
 
 // Staging of the learned end extension (both ends of one sift).
@@ -519,7 +523,8 @@ __device__ __forceinline__ void env_eval_any(const Shared<T, NW>& S, int ev, int
 // of the previous chunk); the first two positions of a segment are decided
 // by fixup() after the pass.
 template <typename T, int NW>
-__device__ __forceinline__ void detect_compact(const Mem<T>& M, int nxt, int64_t cb, int64_t seg_lo,
+__device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M, int nxt,
+                                               int64_t cb, int64_t seg_lo,
                                                const T (&v)[RPL], T& carry1, T& carry2,
                                                int (&run)[2], unsigned long long (&hs)[2],
                                                int trace, int64_t seg_slot) {
@@ -551,24 +556,38 @@ __device__ __forceinline__ void detect_compact(const Mem<T>& M, int nxt, int64_t
   }
   bn0[cb / 8 + ln] = (uint8_t)bmax;
   bn1[cb / 8 + ln] = (uint8_t)bmin;
+  Pair<T>* const st = S.stage[threadIdx.x / WAVE];
 #pragma unroll
   for (int ev = 0; ev < 2; ++ev) {
     const int bits = ev == 0 ? bmax : bmin;
     int tot;
     const int ex = wave_excl_small(__popc(bits), tot);
-    int o = run[ev] + ex;
+    if (tot == 0) continue;  // wave-uniform
+    int o = ex;
     if (bits) {
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
         if ((bits >> k) & 1) {
           const int q = (int)(i0 + k - 1);
-          const T val = (k == 0) ? p1 : v[k - 1];
-          if (o < kcap) put_knot<T>(M.kty + kix(M, nxt, ev, seg_slot + o), q, val);
+          Pair<T> kv;
+          kv.x = (T)q;
+          kv.y = (k == 0) ? p1 : v[k - 1];
+          st[o] = kv;
           if (trace) hs[ev] += mix64((unsigned long long)q);
           ++o;
         }
       }
     }
+    // the wave's own LDS writes above are complete before its reads below
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    Pair<T>* const dst = M.kty + kix(M, nxt, ev, seg_slot + run[ev]);
+    for (int k = ln; k < tot; k += WAVE)
+      if (run[ev] + k < kcap) dst[k] = st[k];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     run[ev] += tot;
   }
 }
@@ -691,7 +710,7 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
             if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)imf[k];
         }
       }
-      detect_compact<T, NW>(M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace, seg_slot);
+      detect_compact<T, NW>(S, M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace, seg_slot);
       A0 = B0;
       A1 = B1;
       cls0 = clsB0;
@@ -762,7 +781,7 @@ __device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, con
           if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)hv[k];
       }
     }
-    detect_compact<T, NW>(M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace,
+    detect_compact<T, NW>(S, M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace,
                           (int64_t)w * M.cap + SEG_FRONT);
   }
   pass_epilogue(S, nxt, w, run, hs, io.trace, false, 0.0, 0.0);
