@@ -481,6 +481,30 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
     const Ivl4 v2 = NC > 2 ? make_ivl(k2, k3) : v1;
     const Ivl4 v3 = NC > 3 ? make_ivl(k3, k4) : v1;
     const Ivl4 v4 = NC > 4 ? make_ivl(k4, k5) : v1;
+    if constexpr (FAST) {
+      // interior chunk: every knot sits on a sample, so t_j - i0 and each
+      // tau = (i0 - t_j) + k are exact; evaluate every candidate interval
+      // and select the value (one select per candidate instead of five)
+      const float fi0 = (float)i0;
+      const float b0 = fi0 - v0.t, b1 = fi0 - v1.t, b2 = fi0 - v2.t, b3 = fi0 - v3.t,
+                  b4 = fi0 - v4.t;
+      auto horner = [](float tau, const Ivl4& iv) {
+        return __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, iv.c3, iv.c2), iv.c1),
+                              iv.y);
+      };
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int j = __popc(c.byte & ((2u << k) - 2u));
+        const float fk = (float)k;
+        float v = horner(b0 + fk, v0);
+        v = (j == 1) ? horner(b1 + fk, v1) : v;
+        if constexpr (NC > 2) v = (j == 2) ? horner(b2 + fk, v2) : v;
+        if constexpr (NC > 3) v = (j == 3) ? horner(b3 + fk, v3) : v;
+        if constexpr (NC > 4) v = (j == 4) ? horner(b4 + fk, v4) : v;
+        acc[k] = ev == 0 ? v : acc[k] + v;
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < RPL; ++k) {
       const int j = FAST ? __popc(c.byte & ((2u << k) - 2u))
