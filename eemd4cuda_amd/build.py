@@ -17,7 +17,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT_DIR = PKG / "_build"
 LIB_NAME = "libeemd_mi355x.so"
-SOURCES = ["eemd_sift.hip", "eemd_mlp.hip", "eemd_capi.cpp"]
+SOURCES = ["eemd_sift.hip", "eemd_mlp.hip", "eemd_spectra.hip", "eemd_capi.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the oracle's rounding (explicit
 # fma() only where intended).  Packed FP32 instructions are disabled.
@@ -26,6 +26,10 @@ HIPCC_FLAGS = [
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+
+
+# hipFFT for the analytic signal of the Hilbert spectrum
+LIBS = ["-L/opt/rocm/lib", "-lhipfft"]
 
 
 def hipcc() -> str:
@@ -45,7 +49,7 @@ def build(verbose: bool = False) -> Path:
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
     tmp = OUT_DIR / (LIB_NAME + ".tmp")
     cmd = [hipcc()] + HIPCC_FLAGS + ["-I", str(CSRC), "-I", str(PKG.parent / "include"),
-                                     "-o", str(tmp)] + srcs
+                                     "-o", str(tmp)] + srcs + LIBS
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

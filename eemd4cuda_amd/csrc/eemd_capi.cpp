@@ -853,3 +853,5 @@ int eemd_last_phases(eemd_ctx* c, double* ms4) {
 }
 
 }  // extern "C"
+
+int eemd::ctx_device(const eemd_ctx* c) { return c ? c->device : 0; }

@@ -17,6 +17,8 @@
 #define __device__
 #endif
 
+typedef struct eemd_ctx eemd_ctx;
+
 namespace eemd {
 
 constexpr int WAVE = 64;
@@ -121,5 +123,8 @@ struct SiftArgs {
   int32_t nslots;           // number of workgroups (scratch slots)
   int32_t member0;          // Philox member index of the first member
 };
+
+// Device of a context (defined with the context in eemd_capi.cpp).
+int ctx_device(const eemd_ctx* c);
 
 }  // namespace eemd

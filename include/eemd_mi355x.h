@@ -171,6 +171,14 @@ int eemd_graph_stats(eemd_ctx* ctx, int64_t* captures, int64_t* launches);
  * dbg_serial_split_v0.py:25-42). */
 int eemd_last_phases(eemd_ctx* ctx, double* ms4);
 
+/* Instantaneous frequency (Hz) and amplitude of each row of x [rows][n]
+ * (host memory), as audio_eemd/spectra.py:32-48 computes them per IMF:
+ * z = x + i H(x) (scipy.signal.hilbert: FFT, one-sided spectrum, inverse
+ * FFT; hipFFT here), inst_freq = np.gradient(np.unwrap(np.angle(z)), 1/sr)
+ * / (2 pi), inst_amp = |z|.  n in [2, 2^27]. */
+int eemd_hilbert_if(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, double sr,
+                    double* inst_freq, double* inst_amp);
+
 /* ------------------------------------------------------------------------
  * Learned boundary extension.  No counterpart in the reference: its GPU
  * kernels use natural ends and its CPU oracle the mirror rule
