@@ -1,4 +1,7 @@
-// Hilbert instantaneous frequency and amplitude of IMFs on gfx950
+// Time-frequency views of IMFs on gfx950: Hilbert instantaneous frequency
+// and amplitude, and the STFT (audio_eemd/spectra.py:18-48).
+//
+// Hilbert instantaneous frequency and amplitude
 // (audio_eemd/spectra.py:32-48: z = imf + i H(imf) by scipy.signal.hilbert,
 // inst_freq = np.gradient(np.unwrap(np.angle(z)), 1/sr) / 2 pi, inst_amp = |z|).
 //
@@ -11,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <climits>
 #include <cmath>
 #include <string>
 
@@ -99,6 +103,33 @@ __global__ void gradient_kernel(const double* u, double* f, int64_t rows, int64_
   f[i] = g / (2.0 * 3.141592653589793);
 }
 
+// frames[r][k][j] = xpad[r][k*step + j] * w[j]; xpad = nperseg/2 zeros, x,
+// nperseg/2 zeros (scipy.signal.stft boundary='zeros'); periodic Hann window.
+__global__ void stft_frames_kernel(const double* x, double* frames, int64_t rows, int64_t n,
+                                   int64_t nseg, int L, int step) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * nseg * L) return;
+  const int j = (int)(i % L);
+  const int64_t k = (i / L) % nseg;
+  const int64_t r = i / ((int64_t)L * nseg);
+  const int64_t src = k * step + j - L / 2;
+  const double v = (src >= 0 && src < n) ? x[r * n + src] : 0.0;
+  const double w = 0.5 - 0.5 * cos(2.0 * 3.141592653589793 * (double)j / (double)L);
+  frames[i] = v * w;
+}
+
+// Zxx[r][f][k] = spec[r][k][f] * scale (complex, interleaved)
+__global__ void stft_layout_kernel(const double2* spec, double2* out, int64_t rows, int64_t nseg,
+                                   int nf, double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * nseg * nf) return;
+  const int64_t k = i % nseg;
+  const int64_t f = (i / nseg) % nf;
+  const int64_t r = i / (nseg * nf);
+  const double2 v = spec[(r * nseg + k) * nf + f];
+  out[i] = make_double2(v.x * scale, v.y * scale);
+}
+
 }  // namespace eemd
 
 using namespace eemd;
@@ -161,6 +192,62 @@ extern "C" int eemd_hilbert_if(eemd_ctx* ctx, const double* x, int64_t rows, int
   }
   if (have_plan) hipfftDestroy(plan);
   for (void* p : {(void*)dx, (void*)dz, (void*)dph, (void*)damp, (void*)dif})
+    if (p) (void)hipFree(p);
+  (void)hipGetLastError();
+  return rc;
+}
+
+extern "C" int eemd_stft(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, int32_t nperseg,
+                         int32_t noverlap, double* zxx) {
+  if (!ctx) return EEMD_ERR_BAD_ARG;
+  if (!x || !zxx || rows < 1 || n < 1 || nperseg < 2 || noverlap < 0 || noverlap >= nperseg)
+    return EEMD_ERR_BAD_ARG;
+  const int L = nperseg, step = nperseg - noverlap, nf = L / 2 + 1;
+  const int64_t len = n + 2 * (int64_t)(L / 2);
+  if (len < L) return EEMD_ERR_BAD_ARG;
+  const int64_t nseg = (len - noverlap) / step;
+  if (nseg < 1 || rows * nseg > (int64_t)INT32_MAX) return EEMD_ERR_BAD_ARG;
+  if (hipSetDevice(ctx_device(ctx)) != hipSuccess) {
+    (void)hipGetLastError();
+    return EEMD_ERR_DEVICE;
+  }
+  double sumw = 0.0;
+  for (int j = 0; j < L; ++j) sumw += 0.5 - 0.5 * std::cos(2.0 * 3.141592653589793 * j / (double)L);
+  double *dx = nullptr, *dfr = nullptr;
+  double2 *dsp = nullptr, *dout = nullptr;
+  hipfftHandle plan = 0;
+  bool have_plan = false;
+  int rc = EEMD_OK;
+  auto chk = [&](hipError_t e) {
+    if (e != hipSuccess && rc == EEMD_OK) rc = (e == hipErrorOutOfMemory) ? EEMD_ERR_OOM : EEMD_ERR_DEVICE;
+    return e == hipSuccess;
+  };
+  const int64_t nfr = rows * nseg;
+  if (chk(hipMalloc(&dx, sizeof(double) * rows * n)) &&
+      chk(hipMalloc(&dfr, sizeof(double) * nfr * L)) &&
+      chk(hipMalloc(&dsp, sizeof(double2) * nfr * nf)) &&
+      chk(hipMalloc(&dout, sizeof(double2) * nfr * nf)) &&
+      chk(hipMemcpy(dx, x, sizeof(double) * rows * n, hipMemcpyHostToDevice))) {
+    int nn = L;
+    if (hipfftPlanMany(&plan, 1, &nn, nullptr, 1, L, nullptr, 1, nf, HIPFFT_D2Z, (int)nfr) !=
+        HIPFFT_SUCCESS) {
+      rc = EEMD_ERR_DEVICE;
+    } else {
+      have_plan = true;
+      hipLaunchKernelGGL(stft_frames_kernel, dim3(nblk(nfr * L, 256)), dim3(256), 0, 0, dx, dfr,
+                         rows, n, nseg, L, step);
+      if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dfr, (hipfftDoubleComplex*)dsp) != HIPFFT_SUCCESS)
+        rc = EEMD_ERR_DEVICE;
+      if (rc == EEMD_OK) {
+        hipLaunchKernelGGL(stft_layout_kernel, dim3(nblk(nfr * nf, 256)), dim3(256), 0, 0, dsp, dout,
+                           rows, nseg, nf, 1.0 / sumw);
+        if (chk(hipGetLastError()) && chk(hipDeviceSynchronize()))
+          chk(hipMemcpy(zxx, dout, sizeof(double2) * nfr * nf, hipMemcpyDeviceToHost));
+      }
+    }
+  }
+  if (have_plan) hipfftDestroy(plan);
+  for (void* p : {(void*)dx, (void*)dfr, (void*)dsp, (void*)dout})
     if (p) (void)hipFree(p);
   (void)hipGetLastError();
   return rc;

@@ -179,6 +179,16 @@ int eemd_last_phases(eemd_ctx* ctx, double* ms4);
 int eemd_hilbert_if(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, double sr,
                     double* inst_freq, double* inst_amp);
 
+/* STFT of each row of x [rows][n] (host memory) as audio_eemd/spectra.py:18-29
+ * asks scipy.signal.stft for it: periodic Hann window of nperseg, noverlap
+ * overlap, nperseg/2 zeros added at both ends, no padding to whole frames,
+ * one-sided spectrum scaled by 1 / sum(window).  zxx: [rows][nperseg/2+1]
+ * [nseg] complex as interleaved (re, im) doubles, nseg = (n + 2*(nperseg/2)
+ * - noverlap) / (nperseg - noverlap); frame k is centred on sample
+ * k*(nperseg-noverlap). */
+int eemd_stft(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, int32_t nperseg,
+              int32_t noverlap, double* zxx);
+
 /* ------------------------------------------------------------------------
  * Learned boundary extension.  No counterpart in the reference: its GPU
  * kernels use natural ends and its CPU oracle the mirror rule

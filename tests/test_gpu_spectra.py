@@ -50,3 +50,20 @@ def test_hilbert_if_of_engine_imfs():
         assert np.mean(np.abs(f[m] - fr)[ok] <= 1e-3) >= 0.999
     st = spectra.if_stats(f[0])
     assert st["median"] is not None and 0 < st["n_pos_frac"] <= 1
+
+
+@pytest.mark.parametrize("n,nperseg,noverlap", [(20000, 4096, None), (5000, 512, 128), (3001, 256, 0)])
+def test_stft_matches_scipy(n, nperseg, noverlap):
+    from eemd4cuda_amd import spectra
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((2, n))
+    print("case: stft n=%d nperseg=%d" % (n, nperseg), flush=True)
+    f, t, Z = spectra.stft_block(x, 48000, nperseg=nperseg, noverlap=noverlap)
+    fr, tr, Zr = ss.stft(x, fs=48000, window="hann", nperseg=nperseg,
+                         noverlap=nperseg // 2 if noverlap is None else noverlap,
+                         boundary="zeros", padded=False)
+    assert Z.shape == Zr.shape
+    assert np.allclose(f, fr, rtol=1e-14, atol=0) and np.allclose(t, tr, rtol=1e-14, atol=1e-15)
+    assert np.max(np.abs(Z - Zr)) <= 1e-12 * np.max(np.abs(Zr))
+    f1, t1, Z1 = spectra.stft_block(x[1], 48000, nperseg=nperseg, noverlap=noverlap)
+    assert np.array_equal(Z1, Z[1])
