@@ -65,7 +65,7 @@ struct Shared {
   // per-wave staging of one envelope's new knots of a chunk (at most 4 per
   // lane: strict extrema are >= 2 samples apart), copied out coalesced
   typename std::conditional<std::is_same<T, float>::value, float2, double2>::type
-      stage[NW][4 * WAVE];
+      stage[NW][4 * WAVE + 1];  // + 1: write target of non-extremum positions
 };
 
 // Staging of the learned end extension (both ends of one sift).
@@ -587,20 +587,21 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
     int tot;
     const int ex = wave_excl_small(__popc(bits), tot);
     if (tot == 0) continue;  // wave-uniform
+    // branch-free: every position writes, non-extrema to the spare entry
     int o = ex;
-    if (bits) {
 #pragma unroll
-      for (int k = 0; k < RPL; ++k) {
-        if ((bits >> k) & 1) {
-          const int q = (int)(i0 + k - 1);
-          Pair<T> kv;
-          kv.x = (T)q;
-          kv.y = (k == 0) ? p1 : v[k - 1];
-          st[o] = kv;
-          if (trace) hs[ev] += mix64((unsigned long long)q);
-          ++o;
-        }
-      }
+    for (int k = 0; k < RPL; ++k) {
+      const int bit = (bits >> k) & 1;
+      Pair<T> kv;
+      kv.x = (T)(int)(i0 + k - 1);
+      kv.y = (k == 0) ? p1 : v[k - 1];
+      st[bit ? o : 4 * WAVE] = kv;
+      o += bit;
+    }
+    if (trace) {
+#pragma unroll
+      for (int k = 0; k < RPL; ++k)
+        if ((bits >> k) & 1) hs[ev] += mix64((unsigned long long)(i0 + k - 1));
     }
     // the wave's own LDS writes above are complete before its reads below
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
