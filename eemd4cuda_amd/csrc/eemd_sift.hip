@@ -1483,25 +1483,24 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
     KRev kr;
     kr_seek(kr, S, M, cur, ev, w1 + 1);
     double xn = xprev;
-    // FP32: owned M values of adjacent slots (s even, s + 1) leave as one
-    // 8-byte store; an odd slot waits one row for its even neighbour
+    // owned M values of adjacent slots (s even, s + 1) leave as one paired
+    // store; an odd slot waits one row for its even neighbour
     int64_t pend_s = -1;
     T pend_v = (T)0;
     auto put_m = [&](int64_t sq, T v) {
-      if constexpr (std::is_same<T, float>::value) {
-        if (pend_s == sq + 1 && (sq & 1) == 0) {
-          *reinterpret_cast<float2*>(mb + sq) = make_float2(v, pend_v);
-          pend_s = -1;
-          return;
-        }
-        if (pend_s >= 0) mb[pend_s] = pend_v;
+      if (pend_s == sq + 1 && (sq & 1) == 0) {
+        T2 mv;
+        mv.x = v;
+        mv.y = pend_v;
+        *reinterpret_cast<T2*>(mb + sq) = mv;
         pend_s = -1;
-        if (sq & 1) {
-          pend_s = sq;
-          pend_v = v;
-        } else {
-          mb[sq] = v;
-        }
+        return;
+      }
+      if (pend_s >= 0) mb[pend_s] = pend_v;
+      pend_s = -1;
+      if (sq & 1) {
+        pend_s = sq;
+        pend_v = v;
       } else {
         mb[sq] = v;
       }
