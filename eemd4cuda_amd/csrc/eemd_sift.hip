@@ -299,15 +299,16 @@ __device__ __forceinline__ double spline_val(double t0, double y0, double m0, do
 
 // Candidate selection by a per-lane index with plain selects.  The
 // candidates are separate scalars (never an array: the compiler would turn a
-// select chain over an array into indexed scratch-memory loads).
+// select chain over an array into indexed scratch-memory loads).  j <= lim
+// is known to the caller; a constant lim drops the impossible candidates.
 template <int NC, typename V>
-__device__ __forceinline__ V sel(int j, V a0, V a1, V a2, V a3, V a4, V a5) {
+__device__ __forceinline__ V sel(int j, V a0, V a1, V a2, V a3, V a4, V a5, int lim = 5) {
   V r = a0;
-  if constexpr (NC > 1) r = (j == 1) ? a1 : r;
-  if constexpr (NC > 2) r = (j == 2) ? a2 : r;
-  if constexpr (NC > 3) r = (j == 3) ? a3 : r;
-  if constexpr (NC > 4) r = (j == 4) ? a4 : r;
-  if constexpr (NC > 5) r = (j == 5) ? a5 : r;
+  if constexpr (NC > 1) if (lim >= 1) r = (j == 1) ? a1 : r;
+  if constexpr (NC > 2) if (lim >= 2) r = (j == 2) ? a2 : r;
+  if constexpr (NC > 3) if (lim >= 3) r = (j == 3) ? a3 : r;
+  if constexpr (NC > 4) if (lim >= 4) r = (j == 4) ? a4 : r;
+  if constexpr (NC > 5) if (lim >= 5) r = (j == 5) ? a5 : r;
   return r;
 }
 
@@ -451,6 +452,93 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   return cls;
 }
 
+// Stage 2, fixed-size form (FP64): as env_prefetch, but every path issues
+// the same six knot loads (surplus ones repeat a valid slot).  With a fixed
+// number of memory operations per chunk, the waits for one chunk's data do
+// not extend to the loads already in flight for the next.  For FP32 the
+// surplus loads cost more than the waits they save.  Returns the wave-uniform candidate class
+// (0: <= 2 intervals per lane, 1: <= 3, 2: <= 5).  Interior chunks (every
+// sample between two real knots, decided for the whole wave from lanes 0 and
+// 63) take the fast path: interval index = popcount of the bits, knot slots
+// consecutive inside the segment run.
+template <typename T, int NW>
+__device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const Mem<T>& M, int cur,
+                                            int ev, int w, int byte, int& crun, int i0,
+                                            EnvChunk<T>& c) {
+  int tot;
+  const int ex = wave_excl_small(__popc(byte), tot);
+  c.byte = byte;
+  c.base = S.pre[cur][ev][w] + crun + ex;
+  crun += tot;
+  c.fast = 0;
+  c.e0 = 0;
+  const int p = S.p[ev];
+  const int64_t rb = kix(M, cur, ev, 0);
+  int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0, x4 = 0, x5 = 0;  // knot slots
+  int cls = 0;
+  if (p >= 2) {
+    const int nl = S.nl[ev];
+    const int cl0 = c.base + (byte & 1);
+    const int cl7 = c.base + __popc(byte);
+    const int lo = __shfl(cl0, 0);
+    const int hi = __shfl(cl7, WAVE - 1);
+    if (lo >= 1 && hi <= p - 1) {
+      c.fast = 1;
+      c.e0 = cl0 + nl - 1;
+      const int span = cl7 - cl0;
+      cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
+      const int g0 = cl0 - 1;
+      int wg = w;
+      const int64_t s0 = real_slot(S, M, cur, ev, g0, wg);
+      const int rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+      const int lastk = span + 1;  // knots g0 .. g0+span+1 are all real
+      auto kk = [&](int k) { return k < lastk ? k : lastk; };
+      if (rem > lastk) {
+        x0 = s0;
+        x1 = s0 + kk(1);
+        x2 = s0 + kk(2);
+        x3 = s0 + kk(3);
+        x4 = s0 + kk(4);
+        x5 = s0 + kk(5);
+      } else {
+        int wh = wg;
+        x0 = s0;
+        x1 = real_slot(S, M, cur, ev, g0 + kk(1), wh);
+        x2 = real_slot(S, M, cur, ev, g0 + kk(2), wh);
+        x3 = real_slot(S, M, cur, ev, g0 + kk(3), wh);
+        x4 = real_slot(S, M, cur, ev, g0 + kk(4), wh);
+        x5 = real_slot(S, M, cur, ev, g0 + kk(5), wh);
+      }
+    } else {
+      const int nr = S.nr[ev], P = S.P[ev];
+      c.e0 = ext_interval(S, ev, cl0, i0, nl, nr, p, P);
+      const int e7 = ext_interval(S, ev, cl7, i0 + RPL - 1, nl, nr, p, P);
+      const int span = e7 - c.e0;
+      cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
+      int wh = w;
+      const int last = P - 1;
+      auto at = [&](int k) {
+        int e = c.e0 + k;
+        e = e > last ? last : e;
+        return ext_slot(S, M, cur, ev, e, wh);
+      };
+      x0 = at(0);
+      x1 = at(1);
+      x2 = at(2);
+      x3 = at(3);
+      x4 = at(4);
+      x5 = at(5);
+    }
+  }
+  load_tym(M, rb + x0, c.t0, c.y0, c.m0);
+  load_tym(M, rb + x1, c.t1, c.y1, c.m1);
+  load_tym(M, rb + x2, c.t2, c.y2, c.m2);
+  load_tym(M, rb + x3, c.t3, c.y3, c.m3);
+  load_tym(M, rb + x4, c.t4, c.y4, c.m4);
+  load_tym(M, rb + x5, c.t5, c.y5, c.m5);
+  return cls;
+}
+
 // Stage 3: values of the envelope at samples i0..i0+7 (written for ev 0,
 // added for ev 1).
 template <int NC, bool FAST, typename T, int NW>
@@ -464,12 +552,14 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
                          : ext_interval(S, ev, c.base + __popc(c.byte & ((2u << k) - 1u)), i0 + k,
                                         nl, nr, p, P) - c.e0;
       const int j1 = j + 1;
-      const double v = spline_val(sel<NC + 1>(j, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5),
-                                  sel<NC + 1>(j, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5),
-                                  sel<NC + 1>(j, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5),
-                                  sel<NC + 1>(j1, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5),
-                                  sel<NC + 1>(j1, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5),
-                                  sel<NC + 1>(j1, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5),
+      // FAST: j counts the knots at positions 1..k, so j <= k
+      const int lj = FAST ? k : 5, lj1 = FAST ? k + 1 : 5;
+      const double v = spline_val(sel<NC + 1>(j, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5, lj),
+                                  sel<NC + 1>(j, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5, lj),
+                                  sel<NC + 1>(j, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5, lj),
+                                  sel<NC + 1>(j1, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5, lj1),
+                                  sel<NC + 1>(j1, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5, lj1),
+                                  sel<NC + 1>(j1, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5, lj1),
                                   (double)(i0 + k));
       acc[k] = ev == 0 ? v : acc[k] + v;
     }
@@ -492,15 +582,17 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
         return __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, iv.c3, iv.c2), iv.c1),
                               iv.y);
       };
+      // j counts the knots at positions 1..k, so sample k has at most k + 1
+      // candidate intervals (k is a constant after unrolling)
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
         const int j = __popc(c.byte & ((2u << k) - 2u));
         const float fk = (float)k;
         float v = horner(b0 + fk, v0);
-        v = (j == 1) ? horner(b1 + fk, v1) : v;
-        if constexpr (NC > 2) v = (j == 2) ? horner(b2 + fk, v2) : v;
-        if constexpr (NC > 3) v = (j == 3) ? horner(b3 + fk, v3) : v;
-        if constexpr (NC > 4) v = (j == 4) ? horner(b4 + fk, v4) : v;
+        if (k >= 1) v = (j == 1) ? horner(b1 + fk, v1) : v;
+        if constexpr (NC > 2) if (k >= 2) v = (j == 2) ? horner(b2 + fk, v2) : v;
+        if constexpr (NC > 3) if (k >= 3) v = (j == 3) ? horner(b3 + fk, v3) : v;
+        if constexpr (NC > 4) if (k >= 4) v = (j == 4) ? horner(b4 + fk, v4) : v;
         acc[k] = ev == 0 ? v : acc[k] + v;
       }
       return;
@@ -647,7 +739,10 @@ __device__ __forceinline__ void pass_epilogue(Shared<T, NW>& S, int nxt, int w,
 
 // SIFT / LAST pass, software-pipelined: while chunk c is evaluated, the
 // bitmap bytes of chunk c+2 and the knots, h (and r) of chunk c+1 are in
-// flight.
+// flight.  FP64 issues the group of loads on every iteration (past the end
+// of the segment it rereads the current chunk and the result goes unused),
+// so its size is the same on every path and waiting for chunk c's data
+// never waits for chunk c+1's; FP32 skips the group past the end.
 template <typename T, int NW, bool LAST>
 __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M, const PassIO& io,
                                                int cur, int nxt) {
@@ -661,6 +756,7 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
   unsigned long long hs[2] = {0ull, 0ull};
   double num = 0.0, den = 0.0;
   T carry1 = (T)0, carry2 = (T)0;
+  constexpr bool FIXED = std::is_same<T, double>::value;
   const bool want_sd = !LAST && io.want_sd;
   const T* const src = io.src_is_r ? M.r : M.h;
   const uint8_t* const bc0 = M.bits + (int64_t)(cur * 2 + 0) * M.bstride;
@@ -672,34 +768,61 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
     int64_t cb = seg_lo;
     const int64_t i00 = cb + (int64_t)ln * RPL;
     int fb0 = 0, fb1 = 0;
-    if (cb + CHUNK < seg_hi) {
-      fb0 = bc0[(cb + CHUNK) / 8 + ln];
-      fb1 = bc1[(cb + CHUNK) / 8 + ln];
+    if (FIXED || cb + CHUNK < seg_hi) {
+      const int64_t cb1 = (cb + CHUNK < seg_hi) ? cb + CHUNK : cb;
+      fb0 = bc0[cb1 / 8 + ln];
+      fb1 = bc1[cb1 / 8 + ln];
     }
     EnvChunk<T> A0, A1;
-    int cls0 = env_prefetch(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
-    int cls1 = env_prefetch(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+    int cls0, cls1;
+    if constexpr (FIXED) {
+      cls0 = env_prefetch_fixed(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
+      cls1 = env_prefetch_fixed(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+    } else {
+      cls0 = env_prefetch(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
+      cls1 = env_prefetch(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+    }
     T hv[RPL], rv[RPL];
     load8(src + i00, hv);
     if constexpr (LAST) load8(M.r + i00, rv);
 #pragma unroll 1
     for (; cb < seg_hi; cb += CHUNK) {
       const int64_t i0 = cb + (int64_t)ln * RPL;
-      const bool has1 = cb + CHUNK < seg_hi;
-      const bool has2 = cb + 2 * CHUNK < seg_hi;
-      int gb0 = 0, gb1 = 0;
-      if (has2) {
-        gb0 = bc0[(cb + 2 * CHUNK) / 8 + ln];
-        gb1 = bc1[(cb + 2 * CHUNK) / 8 + ln];
+      // FP64 LAST: the accumulator values of chunk c are read before the
+      // group below, so that waiting for them does not wait for the group
+      double av[RPL];
+      if constexpr (FIXED && LAST) {
+        const bool rd = io.acc && !io.acc_first;
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) av[k] = (rd && i0 + k < n) ? io.acc[i0 + k] : 0.0;
       }
       EnvChunk<T> B0, B1;
       int clsB0 = 0, clsB1 = 0;
+      int gb0 = 0, gb1 = 0;
       T hn[RPL], rn[RPL];
-      if (has1) {
-        clsB0 = env_prefetch(S, M, cur, 0, w, fb0, crun0, (int)(i0 + CHUNK), B0);
-        clsB1 = env_prefetch(S, M, cur, 1, w, fb1, crun1, (int)(i0 + CHUNK), B1);
-        load8(src + i0 + CHUNK, hn);
-        if constexpr (LAST) load8(M.r + i0 + CHUNK, rn);
+      if constexpr (FIXED) {
+        // chunk c+1 (c+2 for the bitmaps), or chunk c again past the end
+        const int64_t d1 = (cb + CHUNK < seg_hi) ? CHUNK : 0;
+        const int64_t d2 = (cb + 2 * CHUNK < seg_hi) ? 2 * CHUNK : d1;
+        gb0 = bc0[(cb + d2) / 8 + ln];
+        gb1 = bc1[(cb + d2) / 8 + ln];
+        clsB0 = env_prefetch_fixed(S, M, cur, 0, w, fb0, crun0, (int)(i0 + d1), B0);
+        clsB1 = env_prefetch_fixed(S, M, cur, 1, w, fb1, crun1, (int)(i0 + d1), B1);
+        load8(src + i0 + d1, hn);
+        if constexpr (LAST) load8(M.r + i0 + d1, rn);
+      } else {
+        const bool has1 = cb + CHUNK < seg_hi;
+        const bool has2 = cb + 2 * CHUNK < seg_hi;
+        if (has2) {
+          gb0 = bc0[(cb + 2 * CHUNK) / 8 + ln];
+          gb1 = bc1[(cb + 2 * CHUNK) / 8 + ln];
+        }
+        if (has1) {
+          clsB0 = env_prefetch(S, M, cur, 0, w, fb0, crun0, (int)(i0 + CHUNK), B0);
+          clsB1 = env_prefetch(S, M, cur, 1, w, fb1, crun1, (int)(i0 + CHUNK), B1);
+          load8(src + i0 + CHUNK, hn);
+          if constexpr (LAST) load8(M.r + i0 + CHUNK, rn);
+        }
       }
       T msum[RPL];
       env_eval_any(S, 0, cls0, A0, (int)i0, msum);
@@ -732,7 +855,10 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
         if (io.acc) {
 #pragma unroll
           for (int k = 0; k < RPL; ++k)
-            if (i0 + k < n) io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)imf[k];
+            if (i0 + k < n) {
+              if constexpr (FIXED) io.acc[i0 + k] = av[k] + (double)imf[k];
+              else io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)imf[k];
+            }
         }
       }
       detect_compact<T, NW>(S, M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace, seg_slot);
