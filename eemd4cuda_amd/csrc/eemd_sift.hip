@@ -97,6 +97,21 @@ __device__ __forceinline__ int wave_excl_small(int c, int& total) {
   return ex;
 }
 
+// Value of one (wave-uniform) lane broadcast to the wave with v_readlane on
+// each 32-bit word, without a round trip through the LDS permute unit.
+template <typename V>
+__device__ __forceinline__ V lane_val(V v, int lane) {
+  static_assert(sizeof(V) % 4 == 0, "32-bit words");
+  union U {
+    V v;
+    int w[sizeof(V) / 4];
+  } a, b;
+  a.v = v;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(V) / 4); ++i) b.w[i] = __builtin_amdgcn_readlane(a.w[i], lane);
+  return b.v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -372,6 +387,23 @@ __device__ __forceinline__ void load_tym(const Mem<T>& M, int64_t ix, T& t, T& y
   m = M.km[ix];
 }
 
+// The wave's own segment of one envelope of the current set: prefix, next
+// prefix, first slot and knot count.  Constant during a pass, so the slot of
+// a knot inside the segment needs no LDS lookup.
+struct SegCache {
+  int pre, pre1, beg, cnt;
+};
+
+template <typename T, int NW>
+__device__ __forceinline__ SegCache seg_cache(const Shared<T, NW>& S, int set, int ev, int w) {
+  SegCache c;
+  c.pre = S.pre[set][ev][w];
+  c.pre1 = (w < NW - 1) ? S.pre[set][ev][w + 1] : 0x7fffffff;
+  c.beg = S.beg[set][ev][w];
+  c.cnt = S.cnt[set][ev][w];
+  return c;
+}
+
 // Stage 2 of the pipeline for one envelope: interval indices of the chunk
 // and the loads of its knots.  Returns the wave-uniform candidate class
 // (0: <= 2 intervals per lane, 1: <= 3, 2: <= 5).  Interior chunks (every
@@ -380,12 +412,12 @@ __device__ __forceinline__ void load_tym(const Mem<T>& M, int64_t ix, T& t, T& y
 // consecutive inside the segment run.
 template <typename T, int NW>
 __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>& M, int cur,
-                                            int ev, int w, int byte, int& crun, int i0,
-                                            EnvChunk<T>& c) {
+                                            int ev, int w, const SegCache& sc, int byte,
+                                            int& crun, int i0, EnvChunk<T>& c) {
   int tot;
   const int ex = wave_excl_small(__popc(byte), tot);
   c.byte = byte;
-  c.base = S.pre[cur][ev][w] + crun + ex;
+  c.base = sc.pre + crun + ex;
   crun += tot;
   c.fast = 0;
   const int p = S.p[ev];
@@ -406,8 +438,15 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
     const int cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
     const int g0 = cl0 - 1;
     int wg = w;
-    const int64_t s0 = real_slot(S, M, cur, ev, g0, wg);
-    const int rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+    int64_t s0;
+    int rem;
+    if (g0 >= sc.pre && g0 < sc.pre1) {
+      s0 = (int64_t)w * M.cap + sc.beg + (g0 - sc.pre);
+      rem = sc.cnt - (g0 - sc.pre);
+    } else {
+      s0 = real_slot(S, M, cur, ev, g0, wg);
+      rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+    }
     const int need = span + 2;  // knots e0 .. e0+span+1, all real
     if (rem >= need) {
       const int64_t r = rb + s0;
@@ -463,12 +502,12 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
 // consecutive inside the segment run.
 template <typename T, int NW>
 __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const Mem<T>& M, int cur,
-                                            int ev, int w, int byte, int& crun, int i0,
-                                            EnvChunk<T>& c) {
+                                                  int ev, int w, const SegCache& sc, int byte,
+                                                  int& crun, int i0, EnvChunk<T>& c) {
   int tot;
   const int ex = wave_excl_small(__popc(byte), tot);
   c.byte = byte;
-  c.base = S.pre[cur][ev][w] + crun + ex;
+  c.base = sc.pre + crun + ex;
   crun += tot;
   c.fast = 0;
   c.e0 = 0;
@@ -480,8 +519,8 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
     const int nl = S.nl[ev];
     const int cl0 = c.base + (byte & 1);
     const int cl7 = c.base + __popc(byte);
-    const int lo = __shfl(cl0, 0);
-    const int hi = __shfl(cl7, WAVE - 1);
+    const int lo = lane_val(cl0, 0);
+    const int hi = lane_val(cl7, WAVE - 1);
     if (lo >= 1 && hi <= p - 1) {
       c.fast = 1;
       c.e0 = cl0 + nl - 1;
@@ -489,8 +528,15 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
       cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
       const int g0 = cl0 - 1;
       int wg = w;
-      const int64_t s0 = real_slot(S, M, cur, ev, g0, wg);
-      const int rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+      int64_t s0;
+      int rem;
+      if (g0 >= sc.pre && g0 < sc.pre1) {
+        s0 = (int64_t)w * M.cap + sc.beg + (g0 - sc.pre);
+        rem = sc.cnt - (g0 - sc.pre);
+      } else {
+        s0 = real_slot(S, M, cur, ev, g0, wg);
+        rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
+      }
       const int lastk = span + 1;  // knots g0 .. g0+span+1 are all real
       auto kk = [&](int k) { return k < lastk ? k : lastk; };
       if (rem > lastk) {
@@ -656,8 +702,14 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
     p1 = carry1;
     p2 = carry2;
   }
-  carry1 = __shfl(v[RPL - 1], WAVE - 1);
-  carry2 = __shfl(v[RPL - 2], WAVE - 1);
+  // FP64: v_readlane broadcast (measured faster there); FP32: the permute
+  if constexpr (std::is_same<T, double>::value) {
+    carry1 = lane_val(v[RPL - 1], WAVE - 1);
+    carry2 = lane_val(v[RPL - 2], WAVE - 1);
+  } else {
+    carry1 = __shfl(v[RPL - 1], WAVE - 1);
+    carry2 = __shfl(v[RPL - 2], WAVE - 1);
+  }
   const bool first = (cb == seg_lo) && (ln == 0);
   int bmax = 0, bmin = 0;
 #pragma unroll
@@ -767,6 +819,7 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
     int crun0 = 0, crun1 = 0;
     int64_t cb = seg_lo;
     const int64_t i00 = cb + (int64_t)ln * RPL;
+    const SegCache sc0 = seg_cache(S, cur, 0, w), sc1 = seg_cache(S, cur, 1, w);
     int fb0 = 0, fb1 = 0;
     if (FIXED || cb + CHUNK < seg_hi) {
       const int64_t cb1 = (cb + CHUNK < seg_hi) ? cb + CHUNK : cb;
@@ -776,11 +829,11 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
     EnvChunk<T> A0, A1;
     int cls0, cls1;
     if constexpr (FIXED) {
-      cls0 = env_prefetch_fixed(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
-      cls1 = env_prefetch_fixed(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+      cls0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
+      cls1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
     } else {
-      cls0 = env_prefetch(S, M, cur, 0, w, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
-      cls1 = env_prefetch(S, M, cur, 1, w, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+      cls0 = env_prefetch(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
+      cls1 = env_prefetch(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
     }
     T hv[RPL], rv[RPL];
     load8(src + i00, hv);
@@ -806,8 +859,8 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
         const int64_t d2 = (cb + 2 * CHUNK < seg_hi) ? 2 * CHUNK : d1;
         gb0 = bc0[(cb + d2) / 8 + ln];
         gb1 = bc1[(cb + d2) / 8 + ln];
-        clsB0 = env_prefetch_fixed(S, M, cur, 0, w, fb0, crun0, (int)(i0 + d1), B0);
-        clsB1 = env_prefetch_fixed(S, M, cur, 1, w, fb1, crun1, (int)(i0 + d1), B1);
+        clsB0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, fb0, crun0, (int)(i0 + d1), B0);
+        clsB1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, fb1, crun1, (int)(i0 + d1), B1);
         load8(src + i0 + d1, hn);
         if constexpr (LAST) load8(M.r + i0 + d1, rn);
       } else {
@@ -818,8 +871,8 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
           gb1 = bc1[(cb + 2 * CHUNK) / 8 + ln];
         }
         if (has1) {
-          clsB0 = env_prefetch(S, M, cur, 0, w, fb0, crun0, (int)(i0 + CHUNK), B0);
-          clsB1 = env_prefetch(S, M, cur, 1, w, fb1, crun1, (int)(i0 + CHUNK), B1);
+          clsB0 = env_prefetch(S, M, cur, 0, w, sc0, fb0, crun0, (int)(i0 + CHUNK), B0);
+          clsB1 = env_prefetch(S, M, cur, 1, w, sc1, fb1, crun1, (int)(i0 + CHUNK), B1);
           load8(src + i0 + CHUNK, hn);
           if constexpr (LAST) load8(M.r + i0 + CHUNK, rn);
         }
