@@ -387,11 +387,12 @@ __device__ __forceinline__ void load_tym(const Mem<T>& M, int64_t ix, T& t, T& y
   m = M.km[ix];
 }
 
-// The wave's own segment of one envelope of the current set: prefix, next
-// prefix, first slot and knot count.  Constant during a pass, so the slot of
-// a knot inside the segment needs no LDS lookup.
+// The wave's own segment of one envelope of the current set (prefix, next
+// prefix, first slot, knot count) and the envelope's real and left virtual
+// knot counts.  Constant during a pass, so the chunk loop keeps them in
+// registers instead of reading LDS.
 struct SegCache {
-  int pre, pre1, beg, cnt;
+  int pre, pre1, beg, cnt, p, nl;
 };
 
 template <typename T, int NW>
@@ -401,6 +402,8 @@ __device__ __forceinline__ SegCache seg_cache(const Shared<T, NW>& S, int set, i
   c.pre1 = (w < NW - 1) ? S.pre[set][ev][w + 1] : 0x7fffffff;
   c.beg = S.beg[set][ev][w];
   c.cnt = S.cnt[set][ev][w];
+  c.p = S.p[ev];
+  c.nl = S.nl[ev];
   return c;
 }
 
@@ -420,12 +423,12 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   c.base = sc.pre + crun + ex;
   crun += tot;
   c.fast = 0;
-  const int p = S.p[ev];
+  const int p = sc.p;
   if (p < 2) {
     c.e0 = 0;
     return 0;
   }
-  const int nl = S.nl[ev];
+  const int nl = sc.nl;
   const int cl0 = c.base + (byte & 1);
   const int cl7 = c.base + __popc(byte);
   const int lo = __shfl(cl0, 0);
@@ -511,12 +514,12 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
   crun += tot;
   c.fast = 0;
   c.e0 = 0;
-  const int p = S.p[ev];
+  const int p = sc.p;
   const int64_t rb = kix(M, cur, ev, 0);
   int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0, x4 = 0, x5 = 0;  // knot slots
   int cls = 0;
   if (p >= 2) {
-    const int nl = S.nl[ev];
+    const int nl = sc.nl;
     const int cl0 = c.base + (byte & 1);
     const int cl7 = c.base + __popc(byte);
     const int lo = lane_val(cl0, 0);
@@ -661,9 +664,10 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
 }
 
 template <typename T, int NW>
-__device__ __forceinline__ void env_eval_any(const Shared<T, NW>& S, int ev, int cls,
-                                             const EnvChunk<T>& c, int i0, T (&acc)[RPL]) {
-  if (S.p[ev] < 2) {
+__device__ __forceinline__ void env_eval_any(const Shared<T, NW>& S, int ev, const SegCache& sc,
+                                             int cls, const EnvChunk<T>& c, int i0,
+                                             T (&acc)[RPL]) {
+  if (sc.p < 2) {
 #pragma unroll
     for (int k = 0; k < RPL; ++k) acc[k] = ev == 0 ? S.cval[ev] : acc[k] + S.cval[ev];
     return;
@@ -878,8 +882,8 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
         }
       }
       T msum[RPL];
-      env_eval_any(S, 0, cls0, A0, (int)i0, msum);
-      env_eval_any(S, 1, cls1, A1, (int)i0, msum);
+      env_eval_any(S, 0, sc0, cls0, A0, (int)i0, msum);
+      env_eval_any(S, 1, sc1, cls1, A1, (int)i0, msum);
       T v[RPL];
       if constexpr (!LAST) {
 #pragma unroll
