@@ -112,6 +112,22 @@ __device__ __forceinline__ V lane_val(V v, int lane) {
   return b.v;
 }
 
+// Value of the previous lane (lane 0 gets 0) by a whole-wave DPP shift
+// (wave_shr:1), a VALU move instead of an LDS permute.
+template <typename V>
+__device__ __forceinline__ V lane_prev(V v) {
+  static_assert(sizeof(V) % 4 == 0, "32-bit words");
+  union U {
+    V v;
+    int w[sizeof(V) / 4];
+  } a, b;
+  a.v = v;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(V) / 4); ++i)
+    b.w[i] = __builtin_amdgcn_update_dpp(0, a.w[i], 0x138, 0xf, 0xf, false);
+  return b.v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
@@ -700,8 +716,8 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
   const int kcap = (int)(M.cap - SEG_FRONT);
   uint8_t* const bn0 = M.bits + (int64_t)(nxt * 2 + 0) * M.bstride;
   uint8_t* const bn1 = M.bits + (int64_t)(nxt * 2 + 1) * M.bstride;
-  T p1 = __shfl_up(v[RPL - 1], 1);
-  T p2 = __shfl_up(v[RPL - 2], 1);
+  T p1 = lane_prev(v[RPL - 1]);
+  T p2 = lane_prev(v[RPL - 2]);
   if (ln == 0) {
     p1 = carry1;
     p2 = carry2;
