@@ -303,13 +303,17 @@ __device__ __forceinline__ Knot<T> knot_at(const Shared<T, NW>& S, const Mem<T>&
 template <typename T, int NW>
 __device__ __forceinline__ int ext_interval(const Shared<T, NW>& S, int env, int cl, int i, int nl,
                                             int nr, int p, int P) {
+  // virtual knot positions ascend (left: outermost first; right: nearest
+  // first) and usually lie outside the signal: the nearest one settles the
+  // count, the others are compared only when it does not
   int ce = cl + nl;
-  if (cl == 0 && nl > 0) {
+  if (cl == 0 && nl > 0 && !(S.vt[env][nl - 1] < (T)i)) {
     ce = 0;
-    for (int j = 0; j < nl; ++j) ce += (S.vt[env][j] < (T)i) ? 1 : 0;
+    for (int j = 0; j < nl - 1; ++j) ce += (S.vt[env][j] < (T)i) ? 1 : 0;
   }
-  if (cl == p && nr > 0) {
-    for (int j = 0; j < nr; ++j) ce += (S.vt[env][MAX_VIRT + j] < (T)i) ? 1 : 0;
+  if (cl == p && nr > 0 && S.vt[env][MAX_VIRT] < (T)i) {
+    ce += 1;
+    for (int j = 1; j < nr; ++j) ce += (S.vt[env][MAX_VIRT + j] < (T)i) ? 1 : 0;
   }
   int e = ce - 1;
   e = e < 0 ? 0 : e;
@@ -1054,6 +1058,14 @@ __device__ __forceinline__ void prefix(Shared<T, NW>& S, int set) {
 // natural boundary M = 0 at the outermost knots).
 template <typename T, int NW>
 __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, int cur, int end_mode) {
+  // learned ends: when both envelopes have MLP_CTX extrema, learned_ends()
+  // replaces the whole extension, so only the counts are set here; otherwise
+  // the mirror extension stays in place
+  bool counts_only = false;
+  if (end_mode == END_LEARNED) {
+    counts_only = S.pre[cur][0][NW] >= MLP_CTX && S.pre[cur][1][NW] >= MLP_CTX;
+    end_mode = END_MIRROR;
+  }
   for (int ev = 0; ev < 2; ++ev) {
     const int p = S.pre[cur][ev][NW];
     S.p[ev] = p;
@@ -1061,6 +1073,7 @@ __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, in
     S.nr[ev] = 0;
     S.P[ev] = p;
     S.cval[ev] = (T)0;
+    if (counts_only) continue;
     int wh = 0;
     if (p == 1) {
       const Knot<T> k = knot_at(S, M, cur, ev, 0, wh);
@@ -1100,8 +1113,8 @@ __device__ __forceinline__ void setup_ends(Shared<T, NW>& S, const Mem<T>& M, in
 }
 
 // Learned end extension (all threads of the workgroup; call after
-// setup_ends put the mirror extension in place, which stays when an envelope
-// has fewer than MLP_CTX extrema).  For each end, the MLP_CTX extrema nearest
+// setup_ends, which puts the mirror extension in place when an envelope has
+// fewer than MLP_CTX extrema; then this returns and the mirror stays).  For each end, the MLP_CTX extrema nearest
 // to it on both envelopes give the model input:
 //   gaps   d_0 = distance of the nearest extremum from the end sample,
 //          d_i = distance between the i-th and (i-1)-th nearest, as d/G - 1
@@ -1851,7 +1864,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           __syncthreads();
           const int nxt = cur ^ 1;
           if (tid == 0) {
-            setup_ends(S, M, cur, A.end_mode == END_LEARNED ? (int)END_MIRROR : A.end_mode);
+            setup_ends(S, M, cur, A.end_mode);
             S.hsh[nxt][0] = S.hsh[nxt][1] = 0ull;
             if (trace) {
               uint64_t* tr = A.trace + (((int64_t)unit * A.max_imf + m) * A.max_sifts + s) * 4;
