@@ -1988,12 +1988,16 @@ __global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_
   for (int m = 0; m < max_imf; ++m) {
     int cnt = 0;
     double s = 0.0;
+    const int64_t mo = (int64_t)m * sd.n + i;
+    // every task owns max_imf accumulator rows, so the value is loaded
+    // unconditionally (independent loads, no wait on the count first) and a
+    // level the task never reached is dropped by the select
+#pragma unroll 8
     for (int t = sd.task0; t < sd.task0 + sd.ntask; ++t) {
       const int c = acc_cnt[(int64_t)t * max_imf + m];
-      if (c > 0) {
-        s = s + acc[task[t].acc_off + (int64_t)m * sd.n + i];
-        cnt += c;
-      }
+      const double v = acc[task[t].acc_off + mo];
+      s = (c > 0) ? s + v : s;
+      cnt += c;
     }
     if (cnt == 0) break;
     const double a = sums ? s : s / (double)cnt;
