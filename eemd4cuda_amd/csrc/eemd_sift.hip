@@ -518,11 +518,11 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
 // the same six knot loads (surplus ones repeat a valid slot).  With a fixed
 // number of memory operations per chunk, the waits for one chunk's data do
 // not extend to the loads already in flight for the next.  For FP32 the
-// surplus loads cost more than the waits they save.  Returns the wave-uniform candidate class
-// (0: <= 2 intervals per lane, 1: <= 3, 2: <= 5).  Interior chunks (every
-// sample between two real knots, decided for the whole wave from lanes 0 and
-// 63) take the fast path: interval index = popcount of the bits, knot slots
-// consecutive inside the segment run.
+// surplus loads cost more than the waits they save.  Returns the
+// wave-uniform candidate class (0: <= 2 intervals per lane, 1: <= 3, 2: <= 5).
+// Interior chunks (every sample between two real knots, decided for the
+// whole wave from lanes 0 and 63) take the fast path: interval index =
+// popcount of the bits, knot slots consecutive inside the segment run.
 template <typename T, int NW>
 __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const Mem<T>& M, int cur,
                                                   int ev, int w, const SegCache& sc, int byte,
