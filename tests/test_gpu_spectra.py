@@ -67,3 +67,32 @@ def test_stft_matches_scipy(n, nperseg, noverlap):
     assert np.max(np.abs(Z - Zr)) <= 1e-12 * np.max(np.abs(Zr))
     f1, t1, Z1 = spectra.stft_block(x[1], 48000, nperseg=nperseg, noverlap=noverlap)
     assert np.array_equal(Z1, Z[1])
+
+
+def test_artifacts_of_a_decomposition(tmp_path):
+    """The reference's post-decomposition artifacts (audio_eemd/report.py:177-215) written
+    from engine IMFs and GPU spectra, read back with the same keys and values."""
+    import json
+    from eemd4cuda_amd import emd_par, formats, signals, spectra
+    sr = 48000
+    x = signals.audio_mix(1 << 14, seed=2)
+    t = np.arange(len(x)) / sr
+    print("case: artifacts of a decomposition", flush=True)
+    r = emd_par.decompose(x, method="gpu", E=4, tau=0.0, max_sifts=10, max_imf=5, end="mirror",
+                          fp="fp64", noise="philox")
+    p = formats.write_imfs_npz(tmp_path, x, t, r.imfs, r.residual, r.n_modes, r.stats)
+    d = formats.read_imfs_npz(p)
+    assert np.array_equal(d["imfs"], r.imfs) and d["n_modes"] == r.n_modes
+    f, tt, z_imf = spectra.stft_block(r.imfs, sr, nperseg=1024)
+    z_x = spectra.stft_block(x, sr, nperseg=1024)[2]
+    assert z_imf.shape == (r.n_modes,) + z_x.shape
+    f2, t2, zx2, zi2 = formats.read_stft_npz(formats.write_stft_npz(tmp_path, (f, tt, z_x, z_imf)))
+    assert np.array_equal(zi2, z_imf) and np.array_equal(zx2, z_x) and np.array_equal(f2, f)
+    w, a = spectra.hilbert_if(r.imfs, sr)
+    back = formats.read_hilbert_npz(formats.write_hilbert_npz(tmp_path, list(zip(w, a))))
+    assert len(back) == r.n_modes and all(np.array_equal(b[0], w[k]) for k, b in enumerate(back))
+    rows, rrow, q = formats.mode_table(r.imfs, r.residual, x, sr, bit_depth=16)
+    blk = formats.decomposition_block(r.n_modes, r.recon_error(), rows, rrow, q)
+    assert blk["recon_error"] <= 1e-12 and len(blk["imf"]) == r.n_modes
+    rep = json.loads(formats.write_report_json(tmp_path, {"decomposition": blk}).read_text())
+    assert rep["decomposition"]["n_modes"] == r.n_modes
