@@ -399,14 +399,6 @@ struct EnvChunk {
   T t0, y0, m0, t1, y1, m1, t2, y2, m2, t3, y3, m3, t4, y4, m4, t5, y5, m5;
 };
 
-template <typename T>
-__device__ __forceinline__ void load_tym(const Mem<T>& M, int64_t ix, T& t, T& y, T& m) {
-  const Pair<T> v = M.kty[ix];
-  t = v.x;
-  y = v.y;
-  m = M.km[ix];
-}
-
 // The wave's own segment of one envelope of the current set (prefix, next
 // prefix, first slot, knot count) and the envelope's real and left virtual
 // knot counts.  Constant during a pass, so the chunk loop keeps them in
@@ -453,7 +445,18 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   const int cl7 = c.base + __popc(byte);
   const int lo = __shfl(cl0, 0);
   const int hi = __shfl(cl7, WAVE - 1);
+  // knot slots of one (set, envelope) region fit in 32 bits (signals have at
+  // most 2^30 samples, a region about half as many slots): address the
+  // region from its base pointers
   const int64_t rb = kix(M, cur, ev, 0);
+  const Pair<T>* const kt = M.kty + rb;
+  const T* const kmv = M.km + rb;
+  auto ld = [&](int q, T& t, T& y, T& m) {
+    const Pair<T> v = kt[(uint32_t)q];
+    t = v.x;
+    y = v.y;
+    m = kmv[(uint32_t)q];
+  };
   if (lo >= 1 && hi <= p - 1) {
     c.fast = 1;
     c.e0 = cl0 + nl - 1;
@@ -461,33 +464,32 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
     const int cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
     const int g0 = cl0 - 1;
     int wg = w;
-    int64_t s0;
+    int s0;
     int rem;
     if (g0 >= sc.pre && g0 < sc.pre1) {
-      s0 = (int64_t)w * M.cap + sc.beg + (g0 - sc.pre);
+      s0 = w * (int)M.cap + sc.beg + (g0 - sc.pre);
       rem = sc.cnt - (g0 - sc.pre);
     } else {
-      s0 = real_slot(S, M, cur, ev, g0, wg);
+      s0 = (int)real_slot(S, M, cur, ev, g0, wg);
       rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
     }
     const int need = span + 2;  // knots e0 .. e0+span+1, all real
     if (rem >= need) {
-      const int64_t r = rb + s0;
-      load_tym(M, r, c.t0, c.y0, c.m0);
-      load_tym(M, r + 1, c.t1, c.y1, c.m1);
-      if (need > 2) load_tym(M, r + 2, c.t2, c.y2, c.m2);
-      if (cls >= 1 && need > 3) load_tym(M, r + 3, c.t3, c.y3, c.m3);
-      if (cls >= 2 && need > 4) load_tym(M, r + 4, c.t4, c.y4, c.m4);
-      if (cls >= 2 && need > 5) load_tym(M, r + 5, c.t5, c.y5, c.m5);
+      ld(s0, c.t0, c.y0, c.m0);
+      ld(s0 + 1, c.t1, c.y1, c.m1);
+      if (need > 2) ld(s0 + 2, c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) ld(s0 + 3, c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) ld(s0 + 4, c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) ld(s0 + 5, c.t5, c.y5, c.m5);
     } else {
       int wh = wg;
-      auto at = [&](int k) { return rb + real_slot(S, M, cur, ev, g0 + k, wh); };
-      load_tym(M, at(0), c.t0, c.y0, c.m0);
-      load_tym(M, at(1), c.t1, c.y1, c.m1);
-      if (need > 2) load_tym(M, at(2), c.t2, c.y2, c.m2);
-      if (cls >= 1 && need > 3) load_tym(M, at(3), c.t3, c.y3, c.m3);
-      if (cls >= 2 && need > 4) load_tym(M, at(4), c.t4, c.y4, c.m4);
-      if (cls >= 2 && need > 5) load_tym(M, at(5), c.t5, c.y5, c.m5);
+      auto at = [&](int k) { return (int)real_slot(S, M, cur, ev, g0 + k, wh); };
+      ld(at(0), c.t0, c.y0, c.m0);
+      ld(at(1), c.t1, c.y1, c.m1);
+      if (need > 2) ld(at(2), c.t2, c.y2, c.m2);
+      if (cls >= 1 && need > 3) ld(at(3), c.t3, c.y3, c.m3);
+      if (cls >= 2 && need > 4) ld(at(4), c.t4, c.y4, c.m4);
+      if (cls >= 2 && need > 5) ld(at(5), c.t5, c.y5, c.m5);
     }
     return cls;
   }
@@ -501,15 +503,15 @@ __device__ __forceinline__ int env_prefetch(const Shared<T, NW>& S, const Mem<T>
   auto at = [&](int k) {
     int e = c.e0 + k;
     e = e > last ? last : e;
-    return rb + ext_slot(S, M, cur, ev, e, wh);
+    return (int)ext_slot(S, M, cur, ev, e, wh);
   };
-  load_tym(M, at(0), c.t0, c.y0, c.m0);
-  load_tym(M, at(1), c.t1, c.y1, c.m1);
-  load_tym(M, at(2), c.t2, c.y2, c.m2);
-  if (cls >= 1) load_tym(M, at(3), c.t3, c.y3, c.m3);
+  ld(at(0), c.t0, c.y0, c.m0);
+  ld(at(1), c.t1, c.y1, c.m1);
+  ld(at(2), c.t2, c.y2, c.m2);
+  if (cls >= 1) ld(at(3), c.t3, c.y3, c.m3);
   if (cls >= 2) {
-    load_tym(M, at(4), c.t4, c.y4, c.m4);
-    load_tym(M, at(5), c.t5, c.y5, c.m5);
+    ld(at(4), c.t4, c.y4, c.m4);
+    ld(at(5), c.t5, c.y5, c.m5);
   }
   return cls;
 }
@@ -535,8 +537,12 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
   c.fast = 0;
   c.e0 = 0;
   const int p = sc.p;
+  // knot slots of one (set, envelope) region fit in 32 bits (signals have at
+  // most 2^30 samples, a region about half as many slots)
   const int64_t rb = kix(M, cur, ev, 0);
-  int64_t x0 = 0, x1 = 0, x2 = 0, x3 = 0, x4 = 0, x5 = 0;  // knot slots
+  const Pair<T>* const kt = M.kty + rb;
+  const T* const kmv = M.km + rb;
+  int x0 = 0, x1 = 0, x2 = 0, x3 = 0, x4 = 0, x5 = 0;  // knot slots
   int cls = 0;
   if (p >= 2) {
     const int nl = sc.nl;
@@ -551,13 +557,13 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
       cls = (__ballot(span >= 3) != 0ull) ? 2 : ((__ballot(span >= 2) != 0ull) ? 1 : 0);
       const int g0 = cl0 - 1;
       int wg = w;
-      int64_t s0;
+      int s0;
       int rem;
       if (g0 >= sc.pre && g0 < sc.pre1) {
-        s0 = (int64_t)w * M.cap + sc.beg + (g0 - sc.pre);
+        s0 = w * (int)M.cap + sc.beg + (g0 - sc.pre);
         rem = sc.cnt - (g0 - sc.pre);
       } else {
-        s0 = real_slot(S, M, cur, ev, g0, wg);
+        s0 = (int)real_slot(S, M, cur, ev, g0, wg);
         rem = S.cnt[cur][ev][wg] - (g0 - S.pre[cur][ev][wg]);
       }
       const int lastk = span + 1;  // knots g0 .. g0+span+1 are all real
@@ -572,11 +578,11 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
       } else {
         int wh = wg;
         x0 = s0;
-        x1 = real_slot(S, M, cur, ev, g0 + kk(1), wh);
-        x2 = real_slot(S, M, cur, ev, g0 + kk(2), wh);
-        x3 = real_slot(S, M, cur, ev, g0 + kk(3), wh);
-        x4 = real_slot(S, M, cur, ev, g0 + kk(4), wh);
-        x5 = real_slot(S, M, cur, ev, g0 + kk(5), wh);
+        x1 = (int)real_slot(S, M, cur, ev, g0 + kk(1), wh);
+        x2 = (int)real_slot(S, M, cur, ev, g0 + kk(2), wh);
+        x3 = (int)real_slot(S, M, cur, ev, g0 + kk(3), wh);
+        x4 = (int)real_slot(S, M, cur, ev, g0 + kk(4), wh);
+        x5 = (int)real_slot(S, M, cur, ev, g0 + kk(5), wh);
       }
     } else {
       const int nr = S.nr[ev], P = S.P[ev];
@@ -589,7 +595,7 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
       auto at = [&](int k) {
         int e = c.e0 + k;
         e = e > last ? last : e;
-        return ext_slot(S, M, cur, ev, e, wh);
+        return (int)ext_slot(S, M, cur, ev, e, wh);
       };
       x0 = at(0);
       x1 = at(1);
@@ -599,12 +605,18 @@ __device__ __forceinline__ int env_prefetch_fixed(const Shared<T, NW>& S, const 
       x5 = at(5);
     }
   }
-  load_tym(M, rb + x0, c.t0, c.y0, c.m0);
-  load_tym(M, rb + x1, c.t1, c.y1, c.m1);
-  load_tym(M, rb + x2, c.t2, c.y2, c.m2);
-  load_tym(M, rb + x3, c.t3, c.y3, c.m3);
-  load_tym(M, rb + x4, c.t4, c.y4, c.m4);
-  load_tym(M, rb + x5, c.t5, c.y5, c.m5);
+  auto ld = [&](int q, T& t, T& y, T& m) {
+    const Pair<T> v = kt[(uint32_t)q];
+    t = v.x;
+    y = v.y;
+    m = kmv[(uint32_t)q];
+  };
+  ld(x0, c.t0, c.y0, c.m0);
+  ld(x1, c.t1, c.y1, c.m1);
+  ld(x2, c.t2, c.y2, c.m2);
+  ld(x3, c.t3, c.y3, c.m3);
+  ld(x4, c.t4, c.y4, c.m4);
+  ld(x5, c.t5, c.y5, c.m5);
   return cls;
 }
 
