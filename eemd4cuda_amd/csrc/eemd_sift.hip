@@ -727,8 +727,9 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
                                                int (&run)[2], unsigned long long (&hs)[2],
                                                int trace, int64_t seg_slot) {
   const int ln = lane_id();
-  const int64_t n = M.n;
-  const int64_t i0 = cb + (int64_t)ln * RPL;
+  // sample positions fit in 32 bits (signals have at most 2^30 samples)
+  const int n = (int)M.n;
+  const int i0 = (int)cb + ln * RPL;
   const int kcap = (int)(M.cap - SEG_FRONT);
   uint8_t* const bn0 = M.bits + (int64_t)(nxt * 2 + 0) * M.bstride;
   uint8_t* const bn1 = M.bits + (int64_t)(nxt * 2 + 1) * M.bstride;
@@ -750,7 +751,7 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
   int bmax = 0, bmin = 0;
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
-    const int64_t b = i0 + k;
+    const int b = i0 + k;
     const T vm1 = (k == 0) ? p2 : ((k == 1) ? p1 : v[k - 2]);
     const T v0 = (k == 0) ? p1 : v[k - 1];
     const T vp1 = v[k];
@@ -773,7 +774,7 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
     for (int k = 0; k < RPL; ++k) {
       const int bit = (bits >> k) & 1;
       Pair<T> kv;
-      kv.x = (T)(int)(i0 + k - 1);
+      kv.x = (T)(i0 + k - 1);
       kv.y = (k == 0) ? p1 : v[k - 1];
       st[bit ? o : 4 * WAVE] = kv;
       o += bit;
@@ -854,7 +855,8 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
   if (seg_lo < seg_hi) {
     int crun0 = 0, crun1 = 0;
     int64_t cb = seg_lo;
-    const int64_t i00 = cb + (int64_t)ln * RPL;
+    // sample positions fit in 32 bits (signals have at most 2^30 samples)
+    const int i00 = (int)cb + ln * RPL;
     const SegCache sc0 = seg_cache(S, cur, 0, w), sc1 = seg_cache(S, cur, 1, w);
     int fb0 = 0, fb1 = 0;
     if (FIXED || cb + CHUNK < seg_hi) {
@@ -865,25 +867,26 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
     EnvChunk<T> A0, A1;
     int cls0, cls1;
     if constexpr (FIXED) {
-      cls0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
-      cls1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+      cls0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, i00, A0);
+      cls1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, i00, A1);
     } else {
-      cls0 = env_prefetch(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, (int)i00, A0);
-      cls1 = env_prefetch(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, (int)i00, A1);
+      cls0 = env_prefetch(S, M, cur, 0, w, sc0, (int)bc0[cb / 8 + ln], crun0, i00, A0);
+      cls1 = env_prefetch(S, M, cur, 1, w, sc1, (int)bc1[cb / 8 + ln], crun1, i00, A1);
     }
     T hv[RPL], rv[RPL];
-    load8(src + i00, hv);
-    if constexpr (LAST) load8(M.r + i00, rv);
+    load8(src + (uint32_t)i00, hv);
+    if constexpr (LAST) load8(M.r + (uint32_t)i00, rv);
 #pragma unroll 1
     for (; cb < seg_hi; cb += CHUNK) {
-      const int64_t i0 = cb + (int64_t)ln * RPL;
+      const int i0 = (int)cb + ln * RPL;
+      const int ni = (int)n;
       // FP64 LAST: the accumulator values of chunk c are read before the
       // group below, so that waiting for them does not wait for the group
       double av[RPL];
       if constexpr (FIXED && LAST) {
         const bool rd = io.acc && !io.acc_first;
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) av[k] = (rd && i0 + k < n) ? io.acc[i0 + k] : 0.0;
+        for (int k = 0; k < RPL; ++k) av[k] = (rd && i0 + k < ni) ? io.acc[(uint32_t)(i0 + k)] : 0.0;
       }
       EnvChunk<T> B0, B1;
       int clsB0 = 0, clsB1 = 0;
@@ -895,10 +898,10 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
         const int64_t d2 = (cb + 2 * CHUNK < seg_hi) ? 2 * CHUNK : d1;
         gb0 = bc0[(cb + d2) / 8 + ln];
         gb1 = bc1[(cb + d2) / 8 + ln];
-        clsB0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, fb0, crun0, (int)(i0 + d1), B0);
-        clsB1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, fb1, crun1, (int)(i0 + d1), B1);
-        load8(src + i0 + d1, hn);
-        if constexpr (LAST) load8(M.r + i0 + d1, rn);
+        clsB0 = env_prefetch_fixed(S, M, cur, 0, w, sc0, fb0, crun0, i0 + (int)d1, B0);
+        clsB1 = env_prefetch_fixed(S, M, cur, 1, w, sc1, fb1, crun1, i0 + (int)d1, B1);
+        load8(src + (uint32_t)(i0 + (int)d1), hn);
+        if constexpr (LAST) load8(M.r + (uint32_t)(i0 + (int)d1), rn);
       } else {
         const bool has1 = cb + CHUNK < seg_hi;
         const bool has2 = cb + 2 * CHUNK < seg_hi;
@@ -907,27 +910,27 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
           gb1 = bc1[(cb + 2 * CHUNK) / 8 + ln];
         }
         if (has1) {
-          clsB0 = env_prefetch(S, M, cur, 0, w, sc0, fb0, crun0, (int)(i0 + CHUNK), B0);
-          clsB1 = env_prefetch(S, M, cur, 1, w, sc1, fb1, crun1, (int)(i0 + CHUNK), B1);
-          load8(src + i0 + CHUNK, hn);
-          if constexpr (LAST) load8(M.r + i0 + CHUNK, rn);
+          clsB0 = env_prefetch(S, M, cur, 0, w, sc0, fb0, crun0, i0 + CHUNK, B0);
+          clsB1 = env_prefetch(S, M, cur, 1, w, sc1, fb1, crun1, i0 + CHUNK, B1);
+          load8(src + (uint32_t)(i0 + CHUNK), hn);
+          if constexpr (LAST) load8(M.r + (uint32_t)(i0 + CHUNK), rn);
         }
       }
       T msum[RPL];
-      env_eval_any(S, 0, sc0, cls0, A0, (int)i0, msum);
-      env_eval_any(S, 1, sc1, cls1, A1, (int)i0, msum);
+      env_eval_any(S, 0, sc0, cls0, A0, i0, msum);
+      env_eval_any(S, 1, sc1, cls1, A1, i0, msum);
       T v[RPL];
       if constexpr (!LAST) {
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
           v[k] = hv[k] - (T)0.5 * msum[k];
-          if (want_sd && i0 + k < n) {
+          if (want_sd && i0 + k < ni) {
             const T dd = hv[k] - v[k];
             num += (double)(dd * dd);
             den += (double)(hv[k] * hv[k]);
           }
         }
-        store8(M.h + i0, v);
+        store8(M.h + (uint32_t)i0, v);
       } else {
         T imf[RPL];
 #pragma unroll
@@ -935,18 +938,19 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
           imf[k] = hv[k] - (T)0.5 * msum[k];
           v[k] = rv[k] - imf[k];
         }
-        store8(M.r + i0, v);
+        store8(M.r + (uint32_t)i0, v);
         if (io.imf) {
           T* o = reinterpret_cast<T*>(io.imf);
 #pragma unroll
-          for (int k = 0; k < RPL; ++k) if (i0 + k < n) o[i0 + k] = imf[k];
+          for (int k = 0; k < RPL; ++k) if (i0 + k < ni) o[(uint32_t)(i0 + k)] = imf[k];
         }
         if (io.acc) {
 #pragma unroll
           for (int k = 0; k < RPL; ++k)
-            if (i0 + k < n) {
-              if constexpr (FIXED) io.acc[i0 + k] = av[k] + (double)imf[k];
-              else io.acc[i0 + k] = (io.acc_first ? 0.0 : io.acc[i0 + k]) + (double)imf[k];
+            if (i0 + k < ni) {
+              const uint32_t q = (uint32_t)(i0 + k);
+              if constexpr (FIXED) io.acc[q] = av[k] + (double)imf[k];
+              else io.acc[q] = (io.acc_first ? 0.0 : io.acc[q]) + (double)imf[k];
             }
         }
       }
