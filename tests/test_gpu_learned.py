@@ -1,4 +1,4 @@
-"""GPU: the learned end extension and its training kernels.
+"""GPU: the learned end extension and its training kernels (FP64 parity, FP32 one sift).
 
 * Inference: eemd_mlp_infer is bit-identical to the oracle's exact FP32
   inference.
@@ -141,3 +141,40 @@ def test_learned_without_model_is_rejected():
             e2.run_batch(np.sin(np.arange(500) * 0.1), np.array([0, 500]), 1, np.array([0.0]), prm)
     finally:
         e2.close()
+
+
+def test_learned_ends_fp32_one_sift_matches_fp64(eng):
+    """FP32 with learned ends.  For the first sift from x (no noise) both modes
+    see the same knots, so the model input and its virtual extrema agree (the
+    model always runs in FP32); the FP32 sift is then within FP32 rounding of
+    the FP64 one, which is bit-exact with the oracle (test above)."""
+    from eemd4cuda_amd import _lib, learned
+    rng = np.random.default_rng(9)
+    N = 20000
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 300.0) + 0.4 * np.sin(2 * np.pi * t / 23.0) + 0.05 * rng.standard_normal(N)
+    p = mlp_ref.random_params(rng, scale=0.3)
+    learned.install(p, eng)
+    print("case: learned ends fp32 vs fp64 one sift N=20000", flush=True)
+    outs = []
+    for fp64, S in ((1, 30), (0, 14)):
+        prm = _lib.default_params()
+        prm.fp64 = fp64
+        prm.end_mode = _lib.END_MODES["learned"]
+        prm.max_imf = 1
+        prm.max_sifts = 1
+        prm.block = 1024
+        prm.halo = S
+        prm.members_per_task = 1
+        outs.append(eng.run_batch(x, np.array([0, N]), 1, np.array([0.0]), prm,
+                                  noise=np.zeros((1, N)), member_imfs=True))
+    a = np.asarray(outs[0]["member_imfs"][:N], np.float64)
+    b = np.asarray(outs[1]["member_imfs"][:N], np.float64)
+    err = np.max(np.abs(a - b)) / np.max(np.abs(x))
+    print("fp32 vs fp64 learned one-sift max rel err %.2e" % err)
+    assert err <= 2e-5
+    # and the learned ends differ from mirror ends near the ends
+    prm.end_mode = _lib.END_MODES["mirror"]
+    mir = eng.run_batch(x, np.array([0, N]), 1, np.array([0.0]), prm, noise=np.zeros((1, N)),
+                        member_imfs=True)
+    assert not np.array_equal(np.asarray(mir["member_imfs"][:N]), outs[1]["member_imfs"][:N])
