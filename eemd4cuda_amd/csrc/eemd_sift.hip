@@ -1402,7 +1402,7 @@ __device__ __forceinline__ double qdiv(double a, double b) {
 // interface system serially with readlane broadcasts, and every lane
 // reconstructs its rows.  Exact up to rounding (FP32 mode only).
 template <typename T, int NW>
-__device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, int cur, int ev,
+__device__ __noinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, int cur, int ev,
                                             int n, int w0, int w1, int o0, int o1) {
   const int ln = lane_id();
   const int w = (int)(threadIdx.x / WAVE);
@@ -1478,8 +1478,9 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
   double my_a = 0.0, my_b = 0.0, my_A = 0.0, my_B = 0.0;
 #pragma unroll 1
   for (int L = 0; L < nact; ++L) {
-    const double ps = __shfl(Ps, L), qs = __shfl(Qs, L), rs = __shfl(Rs, L);
-    const double pe = __shfl(Pe, L), qe = __shfl(Qe, L), re = __shfl(Re, L);
+    // L is wave-uniform: v_readlane instead of an LDS permute per value
+    const double ps = lane_val(Ps, L), qs = lane_val(Qs, L), rs = lane_val(Rs, L);
+    const double pe = lane_val(Pe, L), qe = lane_val(Qe, L), re = lane_val(Re, L);
     const double inv = __builtin_amdgcn_rcp(1.0 - qs * B);
     const double a = (ps + qs * A) * inv;
     const double b = rs * inv;
@@ -1497,8 +1498,8 @@ __device__ __forceinline__ void window_wave(Shared<T, NW>& S, const Mem<T>& M, i
   double alpha = 0.0, beta = 0.0, Fn = 0.0;
 #pragma unroll 1
   for (int L = nact - 1; L >= 0; --L) {
-    const double fL = __shfl(my_a + my_b * Fn, L);
-    const double gL = __shfl(my_A + my_B * Fn, L);
+    const double fL = lane_val(my_a + my_b * Fn, L);
+    const double gL = lane_val(my_A + my_B * Fn, L);
     if (ln == L) beta = Fn;
     if (ln == L + 1) alpha = gL;
     Fn = fL;
