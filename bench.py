@@ -358,13 +358,21 @@ def main():
         out["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "sift_kernel_ms", "roofline")}
         # FP32 error per ensemble-mean IMF against the FP64 engine (same noise)
         i32, i64 = r32.imfs(), r64.imfs()
-        errs = []
-        for a32, a64 in zip(i32, i64):
+        errs, errs_x = [], []
+        for s_, (a32, a64) in enumerate(zip(i32, i64)):
             m = min(len(a32), len(a64))
+            xn = max(float(np.linalg.norm(wl["xs"][s_])), 1e-300)
             errs.append([float(np.linalg.norm(a32[k] - a64[k]) / max(np.linalg.norm(a64[k]), 1e-300))
                          for k in range(m)])
-        out["fp32_imf_rel_l2_vs_fp64"] = errs[0] if len(errs) == 1 else \
-            [float(np.max([e[k] for e in errs if len(e) > k])) for k in range(max(map(len, errs)))]
+            errs_x.append([float(np.linalg.norm(a32[k] - a64[k]) / xn) for k in range(m)])
+
+        def worst(e):
+            return e[0] if len(e) == 1 else \
+                [float(np.max([v[k] for v in e if len(v) > k])) for k in range(max(map(len, e)))]
+        out["fp32_imf_rel_l2_vs_fp64"] = worst(errs)
+        # the same differences relative to the signal: levels that carry little
+        # energy have large relative errors of small absolute size
+        out["fp32_imf_l2_vs_fp64_rel_to_signal"] = worst(errs_x)
     if a.learned or a.workload == "chirp_scale":
         _log("timing f32 with learned ends")
         out["learned_f32"] = learned_leg(wl, a.steps, a.warmup, a.halo32, f32["value"])
