@@ -6,7 +6,9 @@ seed 1), 256 ensemble members with noise std 0.2*std(x), 10 IMFs x 10 sifts,
 mirror ends, decay-seamed block-Thomas with block 1024.  One "step" is one
 complete EEMD of that record (all members, all IMFs, ensemble mean and
 residual) with the input already resident in HBM and the noise generated
-in-kernel (Philox).  The headline runs in FP32; FP64 is measured alongside.
+in-kernel (Philox).  The headline runs in FP64, the reference's precision,
+bit-exact with the oracle; FP32 is measured alongside with its per-IMF error
+against FP64.
 
 Metric: ensemble sample-sifts/s = sum over members of N x sifts executed,
 divided by the wall time of the timed steps.  roofline.achieved prices the
@@ -16,11 +18,14 @@ reference's CPU path, restated in numpy (oracle/emd_np.py), on a bounded
 sample of members of the same workload, one member per host core.
 
     python bench.py [--steps 5] [--warmup 1] [--workload audio_mix|three_tone|uneven|chirp_scale]
-                    [--learned] [--compare]
+                    [--only f64|f32] [--no-legs] [--no-cpu]
 
-The uneven batch also reports plain per-signal launches against the batched,
-graph-replayed and autotuned launch; the chirp/AM set also reports learned
-ends (throughput cost and edge-region error against mirror ends).
+Besides the headline, the default run reports two bounded legs of the
+metric's own comparison: plain per-signal launches against the batched,
+graph-replayed and autotuned launch on 128 signals of the uneven batch
+(FP64 and FP32), and mirror against learned ends on 512 signals of the
+4096 x 65536 chirp/AM set (FP32: throughput, cost, FP32 error per IMF against
+FP64 on 32 of them, and the edge-region IMF error of 256 held-out windows).
 """
 from __future__ import annotations
 
@@ -119,6 +124,7 @@ class DeviceRun:
         from eemd4cuda_amd import _lib
         self.p.flags = 0
         for s in range(len(self.lens)):
+            self.p.signal0 = s
             o, n = int(self.offs[s]), int(self.lens[s])
             self.eng.run_batch_dev(self.x.data_ptr() + 8 * o, np.array([0, n]), self.E,
                                    self.sig[s:s + 1], self.p,
@@ -127,6 +133,7 @@ class DeviceRun:
                                    self.n_modes.data_ptr() + 4 * s,
                                    self.nmode.data_ptr() + 4 * s * self.E,
                                    self.nsift.data_ptr() + 4 * s * self.E)
+        self.p.signal0 = 0
 
     def autotune(self, reps=1):
         best, ms = self.eng.autotune(self.x.data_ptr(), self.offs, self.E, self.sig, self.p,
@@ -171,25 +178,46 @@ def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0):
                               frac=achieved / HBM_PEAK_GBS, traffic=None)), r
 
 
-def learned_leg(wl, steps, warmup, halo, mirror_value):
-    """FP32 throughput with learned ends (packaged model) on the same
-    workload, its cost against mirror ends, and the edge-region IMF error of
-    mirror vs learned ends on 256 held-out chirp/AM windows (FP64)."""
+def learned_leg(wl, wl_err, steps, warmup, halo):
+    """Learned against mirror ends with the packaged model, FP32: throughput
+    of both on `wl` and the cost of the learned ends; the FP32 error per
+    ensemble-mean IMF against FP64 (both with learned ends, same noise) on
+    `wl_err`; and the edge-region IMF error of mirror vs learned ends on 256
+    held-out chirp/AM windows (FP64, E=1)."""
     import torch
     from eemd4cuda_amd import _lib, learned
     params = learned.load(learned.default_model_path())
     learned.install(params)
-    r = DeviceRun(wl, "f32", halo)
-    r.p.end_mode = _lib.END_MODES["learned"]
-    for _ in range(warmup):
+    out = {"workload": wl["label"], "model": "eemd4cuda_amd/boundary_model.npz"}
+    vals = {}
+    for mode in ("mirror", "learned"):
+        r = DeviceRun(wl, "f32", halo)
+        r.p.end_mode = _lib.END_MODES[mode]
+        for _ in range(warmup):
+            r.step()
+        torch.cuda.synchronize()
+        t = _timed(r, steps, r.step)
+        tm = r.eng.last_timing()
+        ss = r.sample_sifts()
+        vals[mode] = ss / t
+        out[mode] = dict(value=ss / t, ms_per_step=t * 1e3, sift_kernel_ms=tm["sift_ms"],
+                         roofline_frac=ss * BYTES_PER_SAMPLE_SIFT["f32"] / (tm["sift_ms"] / 1e3)
+                         / 1e9 / HBM_PEAK_GBS)
+        del r
+    out["cost_vs_mirror"] = 1.0 - vals["learned"] / vals["mirror"]
+    runs = {}
+    for fp, h in (("f64", 30), ("f32", halo)):
+        r = DeviceRun(wl_err, fp, h)
+        r.p.end_mode = _lib.END_MODES["learned"]
         r.step()
-    torch.cuda.synchronize()
-    t = _timed(r, steps, r.step)
-    ss = r.sample_sifts()
-    v = ss / t
-    ev = learned.edge_error(params, n_windows=256)
-    return dict(value=v, ms_per_step=t * 1e3, cost_vs_mirror=1.0 - v / mirror_value,
-                edge_error=ev, model="eemd4cuda_amd/boundary_model.npz")
+        runs[fp] = r
+    e_imf, e_sig = imf_errors(runs["f32"], runs["f64"], wl_err["xs"])
+    out["fp32_error_workload"] = wl_err["label"] + " learned ends"
+    out["fp32_imf_rel_l2_vs_fp64"] = e_imf
+    out["fp32_imf_l2_vs_fp64_rel_to_signal"] = e_sig
+    del runs
+    out["edge_error"] = learned.edge_error(params, n_windows=256)
+    return out
 
 
 def _timed(r, steps, fn):
@@ -204,15 +232,30 @@ def _timed(r, steps, fn):
 
 def comparators(wl, fp, steps, halo):
     """Plain per-signal launches vs the batched persistent launch, then with
-    hipGraph replay and with the autotuned configuration (same arithmetic)."""
+    hipGraph replay and with the autotuned configuration.  Every launch
+    decomposes the same members (a per-signal call keys its Philox noise by
+    the signal's index in the batch), so in FP64 the member decompositions
+    are identical (checked: sift and IMF counts equal, ensemble means equal
+    up to the order in which member sums are grouped)."""
     r = DeviceRun(wl, fp, halo)
     r.step()
     ss = r.sample_sifts()
+
+    def snap():
+        return (r.avg.cpu().numpy().copy(), r.nsift.cpu().numpy().copy(),
+                r.nmode.cpu().numpy().copy())
+
+    def same_as(ref):
+        a, ns, nm = snap()
+        scale = max(float(np.max(np.abs(ref[0]))), 1e-300)
+        return bool(np.array_equal(ns, ref[1]) and np.array_equal(nm, ref[2]) and
+                    float(np.max(np.abs(a - ref[0]))) <= 1e-12 * scale)
+    ref = snap()
     out = {}
-    t_plain = _timed(r, 1, r.step_per_signal)
-    ss_plain = r.sample_sifts()
-    out["plain_per_signal"] = dict(value=ss_plain / t_plain, ms_per_step=t_plain * 1e3,
-                                   launches=len(r.lens))
+    t_plain = _timed(r, steps, r.step_per_signal)
+    same = same_as(ref)
+    out["plain_per_signal"] = dict(value=ss / t_plain, ms_per_step=t_plain * 1e3,
+                                   launches=len(r.lens), equal_to_batched=same)
     t_b = _timed(r, steps, r.step)
     out["batched"] = dict(value=ss / t_b, ms_per_step=t_b * 1e3)
     r.step(graph=True)
@@ -223,11 +266,18 @@ def comparators(wl, fp, steps, halo):
     tune_s = time.perf_counter() - t0
     r.step(graph=True)   # automatic parameters now resolve to the tuned choice
     t_t = _timed(r, steps, lambda: r.step(graph=True))
+    tm = r.eng.last_timing()
     out["batched_graph_autotuned"] = dict(
         value=ss / t_t, ms_per_step=t_t * 1e3, tune_s=tune_s,
         choice=dict(threads=best.threads, members_per_task=best.members_per_task,
-                    wave_windows=best.wave_windows))
+                    wave_windows=best.wave_windows, block=best.block))
+    if fp == "f64":
+        out["batched_graph_autotuned"]["equal_to_batched"] = same_as(ref)
     out["batched_vs_plain"] = out["batched_graph_autotuned"]["value"] / out["plain_per_signal"]["value"]
+    out["sample_sifts_per_step"] = ss
+    # graph replays are timed as a whole on the engine's stream
+    out["roofline_frac_batched"] = (ss * BYTES_PER_SAMPLE_SIFT[fp] / (tm["sift_ms"] / 1e3) / 1e9
+                                    / HBM_PEAK_GBS)
     return out
 
 
@@ -281,14 +331,51 @@ def copy_probe(mib: int):
     print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
 
 
+def uneven_subset(n: int):
+    """The first n signals of the 512-signal uneven batch (seed 2)."""
+    from eemd4cuda_amd import signals
+    xs = signals.uneven_batch(512, seed=2)[:n]
+    return dict(xs=xs, E=100, max_imf=8, max_sifts=10, B=1024,
+                label="uneven first %d of 512 signals N in [2^12,2^18] E=100 8 IMFs x 10 sifts "
+                      "mirror" % n)
+
+
+def chirp_subset(n: int, E: int = 128):
+    """The first n signals of the 4096 x 65536 chirp/AM set (seed 4)."""
+    from eemd4cuda_amd import signals
+    xs = list(signals.chirp_am_batch(n, 65536, seed=4))
+    return dict(xs=xs, E=E, max_imf=10, max_sifts=10, B=1024,
+                label="chirp_am first %d of 4096 x 65536 E=%d 10 IMFs x 10 sifts" % (n, E))
+
+
+def imf_errors(r_a, r_b, xs):
+    """Worst signal's relative L2 error per ensemble-mean IMF of run a
+    against run b (same noise), relative to the IMF and to the signal."""
+    ia, ib = r_a.imfs(), r_b.imfs()
+    errs, errs_x = [], []
+    for s_, (a_, b_) in enumerate(zip(ia, ib)):
+        m = min(len(a_), len(b_))
+        xn = max(float(np.linalg.norm(xs[s_])), 1e-300)
+        errs.append([float(np.linalg.norm(a_[k] - b_[k]) / max(np.linalg.norm(b_[k]), 1e-300))
+                     for k in range(m)])
+        errs_x.append([float(np.linalg.norm(a_[k] - b_[k]) / xn) for k in range(m)])
+
+    def worst(e):
+        return e[0] if len(e) == 1 else \
+            [float(np.max([v[k] for v in e if len(v) > k])) for k in range(max(map(len, e)))]
+    return worst(errs), worst(errs_x)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="audio_mix")
+    ap.add_argument("--only", choices=("f64", "f32"), default=None,
+                    help="time only this precision on the workload (no legs, no CPU baseline)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-legs", action="store_true", help="skip the comparator and learned legs")
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--phases", action="store_true", help="report per-phase kernel time")
     ap.add_argument("--block", type=int, default=0, help="override the block-Thomas block size")
@@ -298,11 +385,6 @@ def main():
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
-    ap.add_argument("--learned", action="store_true",
-                    help="also time learned ends (packaged model) and report its edge error")
-    ap.add_argument("--compare", action="store_true",
-                    help="also time plain per-signal launches, graph replay and the autotuned "
-                         "configuration (default for the uneven batch)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -323,66 +405,71 @@ def main():
         wl["E"] = a.E
     if a.max_imf:
         wl["max_imf"] = a.max_imf
-    _log("timing f32")
-    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads, mpt=a.mpt,
-                         wave_windows=a.wave_windows)
-    _log("f32 %.1f ms per step" % f32["ms_per_step"])
+    halo = {"f64": 30, "f32": a.halo32}
+    if a.only:
+        res, _ = time_mode(wl, a.only, a.steps, a.warmup, halo=halo[a.only], threads=a.threads,
+                           mpt=a.mpt, wave_windows=a.wave_windows)
+        print(json.dumps(dict(fp=a.only, workload=wl["label"],
+                              **{k: res[k] for k in ("value", "ms_per_step", "sift_kernel_ms",
+                                                     "threads", "grid", "roofline")})),
+              flush=True)
+        return
+    _log("timing f64 (headline)")
+    f64, r64 = time_mode(wl, "f64", a.steps, a.warmup, halo=30, threads=a.threads, mpt=a.mpt)
+    _log("f64 %.1f ms per step" % f64["ms_per_step"])
     out = {
         "metric": "ensemble sample-sifts/s",
-        "value": f32["value"],
+        "value": f64["value"],
         "unit": "sample-sifts/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": f32["ms_per_step"],
+        "ms_per_step": f64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": wl["label"], "fp": "f32 (spline solve and ensemble sum in f64)",
-                   "halo": a.halo32, "block": wl["B"], "threads": f32["threads"], "grid": f32["grid"],
-                   "tasks": f32["ntasks"], "noise": "philox in-kernel",
-                   "sample_sifts_per_step": f32["sample_sifts"]},
-        "roofline": f32["roofline"],
-        "sift_kernel_ms": f32["sift_kernel_ms"],
-        "mean_kernel_ms": f32["mean_kernel_ms"],
+        "config": {"workload": wl["label"], "fp": "f64 (bit-exact with the oracle)",
+                   "halo": 30, "block": wl["B"], "threads": f64["threads"], "grid": f64["grid"],
+                   "tasks": f64["ntasks"], "noise": "philox in-kernel",
+                   "sample_sifts_per_step": f64["sample_sifts"]},
+        "roofline": f64["roofline"],
+        "sift_kernel_ms": f64["sift_kernel_ms"],
+        "mean_kernel_ms": f64["mean_kernel_ms"],
     }
     if a.phases:
+        r64.step(phases=True)
+        out["phases_ms_f64"] = dict(zip(("solve", "pass", "fixup", "other"), r64.eng.last_phases()))
+    _log("timing f32")
+    f32, r32 = time_mode(wl, "f32", a.steps, a.warmup, halo=a.halo32, threads=a.threads,
+                         mpt=a.mpt, wave_windows=a.wave_windows)
+    e_imf, e_sig = imf_errors(r32, r64, wl["xs"])
+    out["fp32"] = {"value": f32["value"], "ms_per_step": f32["ms_per_step"],
+                   "sift_kernel_ms": f32["sift_kernel_ms"], "roofline": f32["roofline"],
+                   "halo": a.halo32, "threads": f32["threads"],
+                   # FP32 error per ensemble-mean IMF against the FP64 run (same
+                   # noise), relative to the IMF and to the signal
+                   "imf_rel_l2_vs_fp64": e_imf, "imf_l2_vs_fp64_rel_to_signal": e_sig}
+    if a.phases:
         r32.step(phases=True)
-        out["phases_ms_f32"] = dict(zip(("solve", "pass", "fixup", "other"), r32.eng.last_phases()))
-    if not a.no_fp64:
-        _log("timing f64")
-        f64, r64 = time_mode(wl, "f64", max(1, a.steps // 2), a.warmup, halo=30,
-                             threads=a.threads, mpt=a.mpt)
-        out["fp64"] = {k: f64[k] for k in ("value", "ms_per_step", "sift_kernel_ms", "roofline")}
-        # FP32 error per ensemble-mean IMF against the FP64 engine (same noise)
-        i32, i64 = r32.imfs(), r64.imfs()
-        errs, errs_x = [], []
-        for s_, (a32, a64) in enumerate(zip(i32, i64)):
-            m = min(len(a32), len(a64))
-            xn = max(float(np.linalg.norm(wl["xs"][s_])), 1e-300)
-            errs.append([float(np.linalg.norm(a32[k] - a64[k]) / max(np.linalg.norm(a64[k]), 1e-300))
-                         for k in range(m)])
-            errs_x.append([float(np.linalg.norm(a32[k] - a64[k]) / xn) for k in range(m)])
-
-        def worst(e):
-            return e[0] if len(e) == 1 else \
-                [float(np.max([v[k] for v in e if len(v) > k])) for k in range(max(map(len, e)))]
-        out["fp32_imf_rel_l2_vs_fp64"] = worst(errs)
-        # the same differences relative to the signal: levels that carry little
-        # energy have large relative errors of small absolute size
-        out["fp32_imf_l2_vs_fp64_rel_to_signal"] = worst(errs_x)
-    if a.learned or a.workload == "chirp_scale":
-        _log("timing f32 with learned ends")
-        out["learned_f32"] = learned_leg(wl, a.steps, a.warmup, a.halo32, f32["value"])
-    if a.compare or a.workload == "uneven":
-        _log("comparators")
-        out["comparators_f32"] = comparators(wl, "f32", max(1, a.steps // 2), a.halo32)
+        out["fp32"]["phases_ms"] = dict(zip(("solve", "pass", "fixup", "other"),
+                                            r32.eng.last_phases()))
+    del r32, r64
     prof = ROOT / "profiles" / ("traffic_%s.json" % a.workload)
     if prof.exists():
         tr = json.loads(prof.read_text())
-        out["roofline"]["traffic"] = tr.get("f32_bytes_per_launch")
+        out["roofline"]["traffic"] = tr.get("f64_bytes_per_launch")
+        out["fp32"]["roofline"]["traffic"] = tr.get("f32_bytes_per_launch")
+    if not a.no_legs and a.workload == "audio_mix":
+        _log("leg: batched vs plain on 128 uneven signals")
+        sub = uneven_subset(128)
+        out["batched_vs_plain"] = {"workload": sub["label"],
+                                   "f64": comparators(sub, "f64", 1, 30),
+                                   "f32": comparators(sub, "f32", 1, a.halo32)}
+        _log("leg: learned vs mirror ends on 512 chirp/AM signals")
+        out["learned_vs_mirror"] = learned_leg(chirp_subset(512), chirp_subset(32), 1, 1,
+                                               a.halo32)
     if not a.no_cpu and len(wl["xs"]) >= 1:
         _log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_cores)

@@ -26,7 +26,7 @@ class Params(C.Structure):
                 ("max_sifts", C.c_int32), ("min_extrema", C.c_int32), ("block", C.c_int32),
                 ("halo", C.c_int32), ("threads", C.c_int32), ("members_per_task", C.c_int32),
                 ("flags", C.c_int32), ("tau", C.c_double), ("seed", C.c_uint64),
-                ("wave_windows", C.c_int32), ("member0", C.c_int32)]
+                ("wave_windows", C.c_int32), ("member0", C.c_int32), ("signal0", C.c_int32)]
 
 
 class EEMDBackendError(RuntimeError):

@@ -102,6 +102,12 @@ struct eemd_ctx {
   int ncu = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // recorded after every call's device work: the next call's stream waits
+  // for it, so calls enqueued on different streams never share the
+  // context's buffers while a previous kernel still uses them (growing a
+  // buffer is safe on its own: hipFree synchronizes the device)
+  hipEvent_t done = nullptr;
+  bool done_recorded = false;
   std::string err;
   // device buffers owned by the context
   DevBuf scratch, acc, acc_cnt, queue, phase;
@@ -120,6 +126,9 @@ struct eemd_ctx {
   bool last_graph = false;
   int64_t graph_captures = 0, graph_launches = 0;
   int last_phase_grid = 0;
+  // state of the spectra entry points (eemd_spectra.hip)
+  void* spectra = nullptr;
+  void (*spectra_free)(void*) = nullptr;
 };
 
 namespace {
@@ -201,6 +210,8 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
                 "FP64 with learned ends needs threads 256 or 512 (LDS holds the model)");
   if (p->member0 < 0 || (int64_t)p->member0 + E > (int64_t(1) << 31))
     return fail(c, EEMD_ERR_BAD_ARG, "member0 must be >= 0 and member0 + E < 2^31");
+  if (p->signal0 < 0 || (int64_t)p->signal0 + nsig > (int64_t(1) << 31))
+    return fail(c, EEMD_ERR_BAD_ARG, "signal0 must be >= 0 and signal0 + nsig < 2^31");
   if (offsets[0] != 0) return fail(c, EEMD_ERR_BAD_ARG, "offsets[0] must be 0");
   for (int s = 0; s < nsig; ++s) {
     const int64_t n = offsets[s + 1] - offsets[s];
@@ -435,11 +446,14 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   A.phase = nullptr;
   A.wave_windows = P.wave_windows;
   A.member0 = p->member0;
+  A.signal0 = p->signal0;
   if (p->flags & EEMD_FLAG_PHASES) {
     HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
     A.phase = (uint64_t*)c->phase.p;
   }
   c->last_phase_grid = (p->flags & EEMD_FLAG_PHASES) ? P.grid : 0;
+  // order this call after the previous one, whatever stream it used
+  if (c->done_recorded) HIPCHK(c, hipStreamWaitEvent(st, c->done, 0), "stream wait");
 
   if (p->flags & EEMD_FLAG_GRAPH) {
     // everything the captured sequence depends on: the plan, the parameters
@@ -456,6 +470,10 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
     key_put(key, P.stride);
     key_put(key, P.wave_windows);
     key_put(key, A);
+    // the mean kernel's outputs are launch arguments outside SiftArgs
+    key_put(key, avg);
+    key_put(key, resid);
+    key_put(key, n_modes);
     GraphCache& G = c->graph;
     if (!G.exec || G.key != key) {
       G.reset();
@@ -492,6 +510,8 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
     HIPCHK(c, enqueue_work(c, P, A, p, c->desc, x, avg, resid, n_modes, st, true), "kernel launch");
     c->last_graph = false;
   }
+  HIPCHK(c, hipEventRecord(c->done, st), "event");
+  c->done_recorded = true;
   c->last_threads = P.threads;
   c->last_grid = P.grid;
   c->last_ntask = (int)nt;
@@ -542,8 +562,8 @@ int eemd_ctx_create(int device, eemd_ctx** out) {
     delete c;
     return EEMD_ERR_DEVICE;
   }
-  for (auto& e : c->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
+  for (hipEvent_t* e : {&c->ev[0], &c->ev[1], &c->ev[2], &c->done}) {
+    if (hipEventCreate(e) != hipSuccess) {
       (void)hipGetLastError();
       delete c;
       return EEMD_ERR_DEVICE;
@@ -559,6 +579,8 @@ int eemd_ctx_destroy(eemd_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->graph.reset();
   c->graph.desc.release();
+  if (c->spectra && c->spectra_free) c->spectra_free(c->spectra);
+  c->spectra = nullptr;
   c->desc.release();
   for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->queue, &c->phase, &c->hx, &c->hnoise,
                     &c->havg, &c->hres, &c->hnm, &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr,
@@ -567,6 +589,7 @@ int eemd_ctx_destroy(eemd_ctx* c) {
     b->release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->done) (void)hipEventDestroy(c->done);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return EEMD_OK;
@@ -715,6 +738,9 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
   int bt = 0, bm = 0, bw = 0, slots = 1;
   double bms = 1e300;
   for (int t : {256, 512, 1024}) {
+    // candidates validate() rejects are skipped (FP64 learned ends: the
+    // model and the 1024-thread workgroup's LDS do not fit together)
+    if (prm->fp64 && prm->end_mode == EEMD_END_LEARNED && t == 1024) continue;
     const int w = prm->fp64 ? -1 : 8 * (t / WAVE);
     double ms;
     rc = timed(t, 0, w, ms, slots);
@@ -855,3 +881,7 @@ int eemd_last_phases(eemd_ctx* c, double* ms4) {
 }  // extern "C"
 
 int eemd::ctx_device(const eemd_ctx* c) { return c ? c->device : 0; }
+void* eemd::ctx_stream(eemd_ctx* c) { return c ? (void*)c->stream : nullptr; }
+int eemd::ctx_fail(eemd_ctx* c, int code, const char* msg) { return fail(c, code, msg); }
+void*& eemd::ctx_spectra(eemd_ctx* c) { return c->spectra; }
+void eemd::ctx_spectra_dtor(eemd_ctx* c, void (*f)(void*)) { c->spectra_free = f; }

@@ -122,9 +122,17 @@ struct SiftArgs {
   int32_t wave_windows;     // FP32: wave-parallel window solve when windows <= this
   int32_t nslots;           // number of workgroups (scratch slots)
   int32_t member0;          // Philox member index of the first member
+  int32_t signal0;          // Philox signal index of the first signal
 };
 
-// Device of a context (defined with the context in eemd_capi.cpp).
+// Context accessors for the other translation units (defined with the
+// context in eemd_capi.cpp): its device, its stream (a hipStream_t), its
+// error message, and a slot for state another unit keeps per context, freed
+// by the given function when the context is destroyed.
 int ctx_device(const eemd_ctx* c);
+void* ctx_stream(eemd_ctx* c);
+int ctx_fail(eemd_ctx* c, int code, const char* msg);
+void*& ctx_spectra(eemd_ctx* c);
+void ctx_spectra_dtor(eemd_ctx* c, void (*f)(void*));
 
 }  // namespace eemd

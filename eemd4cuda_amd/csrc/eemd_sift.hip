@@ -1849,7 +1849,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
       io.z = (A.noise && sd.noise_off >= 0) ? A.noise + sd.noise_off + (int64_t)mem * sd.n : nullptr;
       io.sigma = sd.sigma;
       io.seed = A.seed;
-      io.sig = (uint32_t)td.signal;
+      io.sig = (uint32_t)(A.signal0 + td.signal);
       io.mem = (uint32_t)(A.member0 + mem);
       io.trace = trace;
       __syncthreads();

@@ -138,117 +138,175 @@ namespace {
 
 unsigned nblk(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// Spectra state a context owns: one grow-only device buffer carved into the
+// arrays of a call, and the last batched FFT plan of each entry point (reused
+// while rows and length repeat).  No allocation on repeated calls of one shape.
+struct FftPlan {
+  hipfftHandle h = 0;
+  bool have = false;
+  int64_t rows = 0, n = 0;
+  void reset() {
+    if (have) (void)hipfftDestroy(h);
+    have = false;
+  }
+};
+
+struct SpectraState {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  FftPlan hil, stft;
+};
+
+void spectra_free(void* p) {
+  SpectraState* s = static_cast<SpectraState*>(p);
+  s->hil.reset();
+  s->stft.reset();
+  if (s->buf) (void)hipFree(s->buf);
+  delete s;
+}
+
+SpectraState* state(eemd_ctx* ctx) {
+  void*& slot = ctx_spectra(ctx);
+  if (!slot) {
+    slot = new SpectraState();
+    ctx_spectra_dtor(ctx, spectra_free);
+  }
+  return static_cast<SpectraState*>(slot);
+}
+
+hipError_t ensure(SpectraState* s, size_t want) {
+  if (want <= s->bytes && s->buf) return hipSuccess;
+  if (s->buf) {
+    const hipError_t e = hipFree(s->buf);
+    s->buf = nullptr;
+    s->bytes = 0;
+    if (e != hipSuccess) return e;
+  }
+  const hipError_t e = hipMalloc(&s->buf, want);
+  if (e != hipSuccess) {
+    s->buf = nullptr;
+    return e;
+  }
+  s->bytes = want;
+  return hipSuccess;
+}
+
+size_t al(size_t b) { return (b + 255) & ~size_t(255); }
+
+int fail_hip(eemd_ctx* ctx, hipError_t e, const char* what) {
+  (void)hipGetLastError();
+  return ctx_fail(ctx, e == hipErrorOutOfMemory ? EEMD_ERR_OOM : EEMD_ERR_DEVICE,
+                  (std::string(what) + ": " + hipGetErrorString(e)).c_str());
+}
+
 }  // namespace
 
 extern "C" int eemd_hilbert_if(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, double sr,
                                double* inst_freq, double* inst_amp) {
   if (!ctx) return EEMD_ERR_BAD_ARG;
-  if (!x || !inst_freq || !inst_amp || rows < 1 || n < 2 || !(sr > 0.0) || n > (int64_t(1) << 27))
-    return EEMD_ERR_BAD_ARG;
-  if (hipSetDevice(ctx_device(ctx)) != hipSuccess) {
-    (void)hipGetLastError();
-    return EEMD_ERR_DEVICE;
-  }
+  if (!x || !inst_freq || !inst_amp || rows < 1 || n < 2 || !(sr > 0.0) || n > (int64_t(1) << 27) ||
+      rows > INT_MAX)
+    return ctx_fail(ctx, EEMD_ERR_BAD_ARG, "hilbert: bad arguments");
+  hipError_t e = hipSetDevice(ctx_device(ctx));
+  if (e != hipSuccess) return fail_hip(ctx, e, "set device");
+  hipStream_t st = (hipStream_t)ctx_stream(ctx);
+  SpectraState* S = state(ctx);
   const int64_t total = rows * n;
-  double *dx = nullptr, *dph = nullptr, *damp = nullptr, *dif = nullptr;
-  double2* dz = nullptr;
-  hipfftHandle plan = 0;
-  bool have_plan = false;
-  int rc = EEMD_OK;
-  auto chk = [&](hipError_t e) {
-    if (e != hipSuccess && rc == EEMD_OK) rc = (e == hipErrorOutOfMemory) ? EEMD_ERR_OOM : EEMD_ERR_DEVICE;
-    return e == hipSuccess;
-  };
-  if (chk(hipMalloc(&dx, sizeof(double) * total)) && chk(hipMalloc(&dz, sizeof(double2) * total)) &&
-      chk(hipMalloc(&dph, sizeof(double) * total)) && chk(hipMalloc(&damp, sizeof(double) * total)) &&
-      chk(hipMalloc(&dif, sizeof(double) * total)) &&
-      chk(hipMemcpy(dx, x, sizeof(double) * total, hipMemcpyHostToDevice))) {
+  const size_t bx = al(sizeof(double) * total), bz = al(sizeof(double2) * total);
+  // x, z, phase, amplitude, frequency
+  if ((e = ensure(S, bx + bz + 3 * bx)) != hipSuccess) return fail_hip(ctx, e, "hilbert scratch");
+  char* base = static_cast<char*>(S->buf);
+  double* dx = reinterpret_cast<double*>(base);
+  double2* dz = reinterpret_cast<double2*>(base + bx);
+  double* dph = reinterpret_cast<double*>(base + bx + bz);
+  double* damp = reinterpret_cast<double*>(base + 2 * bx + bz);
+  double* dif = reinterpret_cast<double*>(base + 3 * bx + bz);
+  if (!S->hil.have || S->hil.rows != rows || S->hil.n != n) {
+    S->hil.reset();
     int nn = (int)n;
-    if (hipfftPlanMany(&plan, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_Z2Z, (int)rows) !=
-        HIPFFT_SUCCESS) {
-      rc = EEMD_ERR_DEVICE;
-    } else {
-      have_plan = true;
-      hipLaunchKernelGGL(real_to_complex_kernel, dim3(nblk(total, 256)), dim3(256), 0, 0, dx, dz, total);
-      if (hipfftExecZ2Z(plan, (hipfftDoubleComplex*)dz, (hipfftDoubleComplex*)dz, HIPFFT_FORWARD) !=
-          HIPFFT_SUCCESS)
-        rc = EEMD_ERR_DEVICE;
-      if (rc == EEMD_OK) {
-        hipLaunchKernelGGL(analytic_mask_kernel, dim3(nblk(total, 256)), dim3(256), 0, 0, dz, rows, n);
-        if (hipfftExecZ2Z(plan, (hipfftDoubleComplex*)dz, (hipfftDoubleComplex*)dz,
-                          HIPFFT_BACKWARD) != HIPFFT_SUCCESS)
-          rc = EEMD_ERR_DEVICE;
-      }
-      if (rc == EEMD_OK) {
-        hipLaunchKernelGGL(angle_abs_kernel, dim3(nblk(total, 256)), dim3(256), 0, 0, dz, dph, damp, total);
-        hipLaunchKernelGGL(unwrap_kernel, dim3((unsigned)rows), dim3(1024), 0, 0, dph, n);
-        hipLaunchKernelGGL(gradient_kernel, dim3(nblk(total, 256)), dim3(256), 0, 0, dph, dif, rows, n, sr);
-        if (chk(hipGetLastError()) && chk(hipDeviceSynchronize())) {
-          chk(hipMemcpy(inst_freq, dif, sizeof(double) * total, hipMemcpyDeviceToHost));
-          chk(hipMemcpy(inst_amp, damp, sizeof(double) * total, hipMemcpyDeviceToHost));
-        }
-      }
-    }
+    if (hipfftPlanMany(&S->hil.h, 1, &nn, nullptr, 1, nn, nullptr, 1, nn, HIPFFT_Z2Z, (int)rows) !=
+        HIPFFT_SUCCESS)
+      return ctx_fail(ctx, EEMD_ERR_DEVICE, "hilbert: hipfftPlanMany failed");
+    S->hil.have = true;
+    S->hil.rows = rows;
+    S->hil.n = n;
   }
-  if (have_plan) hipfftDestroy(plan);
-  for (void* p : {(void*)dx, (void*)dz, (void*)dph, (void*)damp, (void*)dif})
-    if (p) (void)hipFree(p);
-  (void)hipGetLastError();
-  return rc;
+  if (hipfftSetStream(S->hil.h, st) != HIPFFT_SUCCESS)
+    return ctx_fail(ctx, EEMD_ERR_DEVICE, "hilbert: hipfftSetStream failed");
+  if ((e = hipMemcpyAsync(dx, x, sizeof(double) * total, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return fail_hip(ctx, e, "hilbert H2D");
+  hipLaunchKernelGGL(real_to_complex_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, dx, dz, total);
+  if (hipfftExecZ2Z(S->hil.h, (hipfftDoubleComplex*)dz, (hipfftDoubleComplex*)dz, HIPFFT_FORWARD) !=
+      HIPFFT_SUCCESS)
+    return ctx_fail(ctx, EEMD_ERR_DEVICE, "hilbert: forward FFT failed");
+  hipLaunchKernelGGL(analytic_mask_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, dz, rows, n);
+  if (hipfftExecZ2Z(S->hil.h, (hipfftDoubleComplex*)dz, (hipfftDoubleComplex*)dz, HIPFFT_BACKWARD) !=
+      HIPFFT_SUCCESS)
+    return ctx_fail(ctx, EEMD_ERR_DEVICE, "hilbert: inverse FFT failed");
+  hipLaunchKernelGGL(angle_abs_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, dz, dph, damp, total);
+  hipLaunchKernelGGL(unwrap_kernel, dim3((unsigned)rows), dim3(1024), 0, st, dph, n);
+  hipLaunchKernelGGL(gradient_kernel, dim3(nblk(total, 256)), dim3(256), 0, st, dph, dif, rows, n, sr);
+  if ((e = hipGetLastError()) != hipSuccess) return fail_hip(ctx, e, "hilbert kernels");
+  if ((e = hipMemcpyAsync(inst_freq, dif, sizeof(double) * total, hipMemcpyDeviceToHost, st)) !=
+      hipSuccess ||
+      (e = hipMemcpyAsync(inst_amp, damp, sizeof(double) * total, hipMemcpyDeviceToHost, st)) !=
+      hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return fail_hip(ctx, e, "hilbert D2H");
+  return EEMD_OK;
 }
 
 extern "C" int eemd_stft(eemd_ctx* ctx, const double* x, int64_t rows, int64_t n, int32_t nperseg,
                          int32_t noverlap, double* zxx) {
   if (!ctx) return EEMD_ERR_BAD_ARG;
   if (!x || !zxx || rows < 1 || n < 1 || nperseg < 2 || noverlap < 0 || noverlap >= nperseg)
-    return EEMD_ERR_BAD_ARG;
+    return ctx_fail(ctx, EEMD_ERR_BAD_ARG, "stft: bad arguments");
   const int L = nperseg, step = nperseg - noverlap, nf = L / 2 + 1;
   const int64_t len = n + 2 * (int64_t)(L / 2);
-  if (len < L) return EEMD_ERR_BAD_ARG;
+  if (len < L) return ctx_fail(ctx, EEMD_ERR_BAD_ARG, "stft: signal shorter than a frame");
   const int64_t nseg = (len - noverlap) / step;
-  if (nseg < 1 || rows * nseg > (int64_t)INT32_MAX) return EEMD_ERR_BAD_ARG;
-  if (hipSetDevice(ctx_device(ctx)) != hipSuccess) {
-    (void)hipGetLastError();
-    return EEMD_ERR_DEVICE;
-  }
+  if (nseg < 1 || rows * nseg > (int64_t)INT32_MAX)
+    return ctx_fail(ctx, EEMD_ERR_BAD_ARG, "stft: too many frames");
+  hipError_t e = hipSetDevice(ctx_device(ctx));
+  if (e != hipSuccess) return fail_hip(ctx, e, "set device");
+  hipStream_t st = (hipStream_t)ctx_stream(ctx);
+  SpectraState* S = state(ctx);
   double sumw = 0.0;
   for (int j = 0; j < L; ++j) sumw += 0.5 - 0.5 * std::cos(2.0 * 3.141592653589793 * j / (double)L);
-  double *dx = nullptr, *dfr = nullptr;
-  double2 *dsp = nullptr, *dout = nullptr;
-  hipfftHandle plan = 0;
-  bool have_plan = false;
-  int rc = EEMD_OK;
-  auto chk = [&](hipError_t e) {
-    if (e != hipSuccess && rc == EEMD_OK) rc = (e == hipErrorOutOfMemory) ? EEMD_ERR_OOM : EEMD_ERR_DEVICE;
-    return e == hipSuccess;
-  };
   const int64_t nfr = rows * nseg;
-  if (chk(hipMalloc(&dx, sizeof(double) * rows * n)) &&
-      chk(hipMalloc(&dfr, sizeof(double) * nfr * L)) &&
-      chk(hipMalloc(&dsp, sizeof(double2) * nfr * nf)) &&
-      chk(hipMalloc(&dout, sizeof(double2) * nfr * nf)) &&
-      chk(hipMemcpy(dx, x, sizeof(double) * rows * n, hipMemcpyHostToDevice))) {
+  const size_t bx = al(sizeof(double) * rows * n), bfr = al(sizeof(double) * nfr * L),
+               bsp = al(sizeof(double2) * nfr * nf);
+  // x, frames, spectra, output
+  if ((e = ensure(S, bx + bfr + 2 * bsp)) != hipSuccess) return fail_hip(ctx, e, "stft scratch");
+  char* base = static_cast<char*>(S->buf);
+  double* dx = reinterpret_cast<double*>(base);
+  double* dfr = reinterpret_cast<double*>(base + bx);
+  double2* dsp = reinterpret_cast<double2*>(base + bx + bfr);
+  double2* dout = reinterpret_cast<double2*>(base + bx + bfr + bsp);
+  if (!S->stft.have || S->stft.rows != nfr || S->stft.n != L) {
+    S->stft.reset();
     int nn = L;
-    if (hipfftPlanMany(&plan, 1, &nn, nullptr, 1, L, nullptr, 1, nf, HIPFFT_D2Z, (int)nfr) !=
-        HIPFFT_SUCCESS) {
-      rc = EEMD_ERR_DEVICE;
-    } else {
-      have_plan = true;
-      hipLaunchKernelGGL(stft_frames_kernel, dim3(nblk(nfr * L, 256)), dim3(256), 0, 0, dx, dfr,
-                         rows, n, nseg, L, step);
-      if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dfr, (hipfftDoubleComplex*)dsp) != HIPFFT_SUCCESS)
-        rc = EEMD_ERR_DEVICE;
-      if (rc == EEMD_OK) {
-        hipLaunchKernelGGL(stft_layout_kernel, dim3(nblk(nfr * nf, 256)), dim3(256), 0, 0, dsp, dout,
-                           rows, nseg, nf, 1.0 / sumw);
-        if (chk(hipGetLastError()) && chk(hipDeviceSynchronize()))
-          chk(hipMemcpy(zxx, dout, sizeof(double2) * nfr * nf, hipMemcpyDeviceToHost));
-      }
-    }
+    if (hipfftPlanMany(&S->stft.h, 1, &nn, nullptr, 1, L, nullptr, 1, nf, HIPFFT_D2Z, (int)nfr) !=
+        HIPFFT_SUCCESS)
+      return ctx_fail(ctx, EEMD_ERR_DEVICE, "stft: hipfftPlanMany failed");
+    S->stft.have = true;
+    S->stft.rows = nfr;
+    S->stft.n = L;
   }
-  if (have_plan) hipfftDestroy(plan);
-  for (void* p : {(void*)dx, (void*)dfr, (void*)dsp, (void*)dout})
-    if (p) (void)hipFree(p);
-  (void)hipGetLastError();
-  return rc;
+  if (hipfftSetStream(S->stft.h, st) != HIPFFT_SUCCESS)
+    return ctx_fail(ctx, EEMD_ERR_DEVICE, "stft: hipfftSetStream failed");
+  if ((e = hipMemcpyAsync(dx, x, sizeof(double) * rows * n, hipMemcpyHostToDevice, st)) != hipSuccess)
+    return fail_hip(ctx, e, "stft H2D");
+  hipLaunchKernelGGL(stft_frames_kernel, dim3(nblk(nfr * L, 256)), dim3(256), 0, st, dx, dfr, rows,
+                     n, nseg, L, step);
+  if (hipfftExecD2Z(S->stft.h, (hipfftDoubleReal*)dfr, (hipfftDoubleComplex*)dsp) != HIPFFT_SUCCESS)
+    return ctx_fail(ctx, EEMD_ERR_DEVICE, "stft: FFT failed");
+  hipLaunchKernelGGL(stft_layout_kernel, dim3(nblk(nfr * nf, 256)), dim3(256), 0, st, dsp, dout,
+                     rows, nseg, nf, 1.0 / sumw);
+  if ((e = hipGetLastError()) != hipSuccess) return fail_hip(ctx, e, "stft kernels");
+  if ((e = hipMemcpyAsync(zxx, dout, sizeof(double2) * nfr * nf, hipMemcpyDeviceToHost, st)) !=
+      hipSuccess ||
+      (e = hipStreamSynchronize(st)) != hipSuccess)
+    return fail_hip(ctx, e, "stft D2H");
+  return EEMD_OK;
 }

@@ -136,13 +136,21 @@ def _gpu_decompose(x, E, eps, tau, max_sifts, max_imf, min_extrema, end, seed, b
     if reached:
         warnings.warn("max_imf=-1 is capped at 64 IMFs on the GPU and the cap was reached: the "
                       "residual may still hold oscillations", RuntimeWarning, stacklevel=3)
+    sample_sifts = int(nsift.sum()) * N
     stats = dict(method="gpu", E=E, n_modes=n_modes, batch=batch,
                  per_trial_total_sifts=nsift.astype(np.int64), gpu_time_s=dt, x=x,
                  total_sifts=int(nsift.sum()), fp_mode=fp, boundary=end,
                  B=int(B), S=int(S), wg=tm["threads"], grid=tm["grid"], tasks=tm["ntasks"],
+                 # members one workgroup decomposes per queue entry (last batch)
+                 members_per_wg=-(-min(batch, E) // max(int(tm["ntasks"]), 1)),
                  sift_kernel_ms=tm["sift_ms"], mean_kernel_ms=tm["mean_ms"],
-                 sample_sifts=int(nsift.sum()) * N, noise=noise,
-                 n_batches=(E + batch - 1) // batch, max_imf_reached=reached)
+                 sample_sifts=sample_sifts,
+                 # algorithmic HBM bytes of the sift kernels: per sample-sift one
+                 # read and one write of h, plus per IMF the residual update and
+                 # the FP64 ensemble accumulation, over 10 sifts per IMF
+                 hbm_bytes=int(sample_sifts * (19.2 if fp == "fp64" else 10.4)),
+                 noise=noise, n_batches=(E + batch - 1) // batch, max_imf_reached=reached,
+                 per_trial_per_mode_sifts=np.concatenate(per_mode_all)[:, :n_modes].astype(np.int64))
     if E == 1:
         stats["per_mode_sifts"] = [int(v) for v in per_mode_all[0][0, :n_modes]]
     return avg, resid, stats

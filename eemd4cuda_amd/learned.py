@@ -336,30 +336,44 @@ def run(x, offsets, E, sigmas, prm, model=None, noise=None, engine=None, **kw):
 
 
 def edge_error(params, n_windows: int = 256, W: int = 4096, L: int = 16384, seed: int = 5,
-               max_imf: int = 3, fp64: int = 1, engine=None):
+               max_imf: int = 3, fp64: int = 1, E: int = 1, eps: float = 0.0, engine=None):
     """Relative L2 error of window IMFs against the long signal's IMFs over
     the edge regions (first and last ceil(W/16) samples), mirror vs learned.
 
-    Windows come from fresh long signals (seed `seed`).  Each window is
-    decomposed alone (E=1, no noise, 10 sifts, FP64), with mirror and with
-    learned ends.  The truth is the long signal's decomposition restricted to
-    the window.  Returns per-IMF errors for both end modes and their ratio.
+    Windows come from fresh long signals (seed `seed`), four per signal.  The
+    truth is the long signal's FP64 decomposition (mirror ends, 10 sifts)
+    restricted to the window.  Each window is decomposed alone (10 sifts;
+    FP64 or FP32 by `fp64`) with mirror and with learned ends.  With E > 1
+    both are EEMDs with E members: member e of a window receives the long
+    signal's member-e noise restricted to the window (noise std eps * std of
+    the long signal), so the window ensemble differs from the truth only by
+    what its ends do.  Returns per-IMF errors (aggregate over the windows and
+    the median window) for both end modes and their ratios.
     """
     eng = engine or _lib.engine()
     rng = np.random.default_rng(seed)
     n_sig = max(1, n_windows // 4)
     xs = np.stack([signals.chirp_am(L, rng) for _ in range(n_sig)])
+    noisy = E > 1 or eps > 0.0
+    z = rng.standard_normal((n_sig, E, L)) if noisy else None
+    sig_long = eps * np.std(xs, axis=1) if noisy else np.zeros(n_sig)
     prm = _lib.default_params()
     prm.fp64 = 1
     prm.end_mode = _lib.END_MODES["mirror"]
     prm.max_imf = max_imf
     prm.max_sifts = 10
     offs = np.arange(n_sig + 1, dtype=np.int64) * L
-    full = eng.run_batch(xs.ravel(), offs, 1, np.zeros(n_sig), prm, member_imfs=True)
-    imf_long = full["member_imfs"].reshape(n_sig, max_imf, L)
+    full = eng.run_batch(xs.ravel(), offs, E, sig_long, prm,
+                         noise=None if z is None else z.ravel())
+    imf_long = full["avg"].reshape(n_sig, max_imf, L)
     starts = rng.integers(1024, L - W - 1024, (n_sig, 4))
     wins = np.stack([xs[s, a:a + W] for s in range(n_sig) for a in starts[s]])
     truth = np.stack([imf_long[s, :, a:a + W] for s in range(n_sig) for a in starts[s]])
+    wz = None
+    wsig = np.zeros(len(wins))
+    if noisy:
+        wz = np.stack([z[s, :, a:a + W] for s in range(n_sig) for a in starts[s]])
+        wsig = np.repeat(sig_long, 4)
     ew = -(-W // 16)
     edge = np.zeros(W, bool)
     edge[:ew] = True
@@ -376,8 +390,9 @@ def edge_error(params, n_windows: int = 256, W: int = 4096, L: int = 16384, seed
         if mode == "learned":
             install(params, eng)
         p.end_mode = _lib.END_MODES[mode]
-        o = eng.run_batch(wins.ravel(), woffs, 1, np.zeros(len(wins)), p, member_imfs=True)
-        got = o["member_imfs"].reshape(len(wins), max_imf, W)
+        o = eng.run_batch(wins.ravel(), woffs, E, wsig, p,
+                          noise=None if wz is None else wz.ravel())
+        got = o["avg"].reshape(len(wins), max_imf, W)
         errs, med = [], []
         for m in range(max_imf):
             d2 = np.sum((got[:, m, edge] - truth[:, m, edge]) ** 2, axis=1)
@@ -391,6 +406,8 @@ def edge_error(params, n_windows: int = 256, W: int = 4096, L: int = 16384, seed
                                   zip(res["learned_median_window"], res["mirror_median_window"])]
     res["edge_samples"] = ew
     res["windows"] = len(wins)
+    res["members"] = E
+    res["fp"] = "f64" if fp64 else "f32"
     return res
 
 

@@ -79,10 +79,11 @@ typedef struct eemd_params {
   int32_t wave_windows;     /* FP32: solve windows a wave at a time when a workgroup has at most
                                this many (0: automatic, < 0: never) */
   int32_t member0;          /* Philox member index of member 0 (batches of one ensemble) */
+  int32_t signal0;          /* Philox signal index of signal 0 (a batch split over calls) */
 } eemd_params;
 
 /* Defaults: FP64, mirror, max_imf 10, max_sifts 10, min_extrema 4, B 1024, S 30,
- * automatic workgroup / task sizes, tau 0, seed 0, member0 0. */
+ * automatic workgroup / task sizes, tau 0, seed 0, member0 0, signal0 0. */
 void eemd_params_default(eemd_params* p);
 
 int eemd_ctx_create(int device, eemd_ctx** out);
@@ -94,7 +95,8 @@ const char* eemd_last_error(const eemd_ctx* ctx);
  * offsets[s+1]), length >= 3) with E members each.  Member e of signal s
  * decomposes x_s + sigma_s * z_{s,e}, where z is the explicit standard-normal
  * matrix `noise` (per signal an [E][n_s] block, blocks concatenated in signal
- * order) or, when noise is NULL, the Philox normals of eemd_noise().
+ * order) or, when noise is NULL, the Philox normals of eemd_noise() for signal
+ * index signal0 + s and members member0 ...
  *
  * Outputs (host memory, caller-owned):
  *   imf_mean   [sum_s max_imf * n_s]  per signal a [max_imf][n_s] block; levels

@@ -3,6 +3,7 @@ entry point include/*.h declares, and the Python surface validates its inputs
 the way the reference does (emd_par.py:281-299, eemd_stage.py:66-69)."""
 import ctypes
 import re
+import subprocess
 from pathlib import Path
 
 import numpy as np
@@ -43,9 +44,45 @@ def test_params_default_without_gpu():
         (1, 1, 10, 10, 4, 1024, 30)
 
 
-def test_params_struct_layout_matches_header():
+def _c_layout(tmp_path, fields):
+    """sizeof(eemd_params) and offsetof of each field, from the header by gcc."""
+    src = tmp_path / "layout.c"
+    body = "".join('  printf("%%zu\\n", offsetof(eemd_params, %s));\n' % f for f in fields)
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "eemd_mi355x.h"\n'
+                   'int main(void) {\n  printf("%zu\\n", sizeof(eemd_params));\n' + body +
+                   '  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), "-o", str(exe), str(src)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                           text=True).stdout.split()]
+    return vals[0], vals[1:]
+
+
+def test_params_struct_layout_matches_header(tmp_path):
     from eemd4cuda_amd import _lib
-    assert ctypes.sizeof(_lib.Params) == 64
+    names = [f[0] for f in _lib.Params._fields_]
+    size, offs = _c_layout(tmp_path, names)
+    assert ctypes.sizeof(_lib.Params) == size
+    assert [getattr(_lib.Params, n).offset for n in names] == offs
+
+
+def test_integration_stub_params_match_header(tmp_path):
+    """The eemd_params mirror in INTEGRATION.md's ctypes stub has the header's layout."""
+    ns = {}
+    exec(_integration_params_source(), ns)
+    P = ns["_Params"]
+    names = [f[0] for f in P._fields_]
+    size, offs = _c_layout(tmp_path, names)
+    assert ctypes.sizeof(P) == size
+    assert [getattr(P, n).offset for n in names] == offs
+
+
+def _integration_params_source():
+    txt = (ROOT / "INTEGRATION.md").read_text()
+    block = re.search(r"```python\n(import ctypes as C.*?)```", txt, flags=re.S).group(1)
+    a = block.index("class _Params")
+    b = block.index("\n\n", a)
+    return "import ctypes as C\n" + block[a:b] + "\n"
 
 
 def test_decompose_input_validation():

@@ -59,6 +59,15 @@ def test_decompose_matches_reference_host_driver(two_tones):
     assert r.n_modes == o["n_modes"]
     assert np.max(np.abs(r.imfs - o["avg"])) <= 1e-12
     assert r.recon_error() <= 1e-12
+    st = r.stats
+    for k in ("method", "E", "n_modes", "batch", "per_trial_total_sifts", "gpu_time_s", "x",
+              "total_sifts", "max_imf_reached", "fp_mode", "boundary", "B", "S", "wg",
+              "members_per_wg", "hbm_bytes", "sample_sifts"):
+        assert k in st, k
+    assert st["sample_sifts"] == int(np.sum(o["nsift"])) * len(x)
+    assert st["hbm_bytes"] == int(st["sample_sifts"] * 19.2)
+    assert 1 <= st["members_per_wg"] <= 4
+    assert np.array_equal(st["per_trial_per_mode_sifts"], o["per_mode"][:, :r.n_modes])
 
 
 def test_gpu_eemd_full_single_sift():
@@ -125,3 +134,30 @@ def test_decompose_auto_batch_uses_free_memory():
     r = emd_par.decompose(x, method="gpu", E=4, max_imf=-1, tau=0.0, max_sifts=10, end="mirror")
     assert r.stats["batch"] == 4        # four members fit at once
     assert r.stats["max_imf_reached"] is False
+
+
+def test_integration_stub_runs_and_matches_oracle():
+    """The ctypes stub of INTEGRATION.md, executed as written (library path
+    filled in): its EEMD equals the oracle with the reference's DSBT geometry
+    B=250, S=30."""
+    import re
+    from pathlib import Path
+
+    from eemd4cuda_amd.build import lib_path
+    txt = (Path(__file__).resolve().parents[1] / "INTEGRATION.md").read_text()
+    code = re.search(r"```python\n(import ctypes as C.*?)```", txt, flags=re.S).group(1)
+    code = code.replace('C.CDLL("libeemd_mi355x.so")', "C.CDLL(%r)" % str(lib_path()))
+    ns = {}
+    exec(compile(code, "INTEGRATION.md", "exec"), ns)
+    rng = np.random.default_rng(11)
+    N, E = 5000, 4
+    x = np.sin(np.arange(N) * 0.02) + 0.4 * np.sin(np.arange(N) * 0.31) + 0.1 * rng.standard_normal(N)
+    noise = np.random.default_rng(0).standard_normal((E, N))
+    sigma = 0.2 * np.std(x)
+    print("case: INTEGRATION.md stub", flush=True)
+    imfs, resid, k, nsift, nmode = ns["_mi355x_eemd"](x, noise, sigma, 0.25, 50, 6, end="natural")
+    o = orc.eemd(x, noise, sigma, tau=0.25, max_sifts=50, max_imf=6, end="natural", B=250, S=30)
+    assert k == o["n_modes"]
+    assert np.array_equal(nsift, o["nsift"]) and np.array_equal(nmode, o["nmode"])
+    assert np.max(np.abs(imfs - o["avg"])) <= 1e-12
+    assert np.max(np.abs(resid - o["resid"])) <= 1e-12

@@ -138,3 +138,115 @@ def test_autotune_choice_is_applied_and_exact(eng, fp64):
         assert np.array_equal(exp["nsift"], base["nsift"])
         scale = np.max(np.abs(base["avg"]))
         assert np.max(np.abs(exp["avg"] - base["avg"])) <= 1e-13 * scale
+
+
+def test_graph_new_output_buffer_recaptures(eng):
+    """Only the mean output buffer changes between two graph calls: the call
+    is captured again and fills the new buffer (the old one is left alone)."""
+    import torch
+    from eemd4cuda_amd import _lib
+    x, offs, sig = _batch(6, [6000, 3000])
+    E, mi = 4, 6
+    dev = torch.device("cuda", 0)
+    xd = torch.from_numpy(x).to(dev)
+    n = len(x)
+    res = torch.zeros(n, dtype=torch.float64, device=dev)
+    nm = torch.zeros(2, dtype=torch.int32, device=dev)
+    nmode = torch.zeros(2 * E, dtype=torch.int32, device=dev)
+    nsift = torch.zeros(2 * E, dtype=torch.int32, device=dev)
+    p = _params(1, max_imf=mi)
+    ref = _dev_run(eng, xd, offs, E, sig, p, mi)
+    avgs = []
+    for k in range(2):
+        avg = torch.full((mi * n,), -1.0, dtype=torch.float64, device=dev)
+        cap0, _ = eng.graph_stats()
+        p.flags = _lib.FLAG_GRAPH
+        print("case: graph with output buffer", k, flush=True)
+        eng.run_batch_dev(xd.data_ptr(), offs, E, sig, p, avg.data_ptr(), res.data_ptr(),
+                          nm.data_ptr(), nmode.data_ptr(), nsift.data_ptr())
+        torch.cuda.synchronize()
+        cap1, _ = eng.graph_stats()
+        assert cap1 == cap0 + 1
+        avgs.append(avg)
+    assert np.array_equal(avgs[1].cpu().numpy(), ref["avg"])
+    # the first buffer holds the first call's output and was not written again
+    avgs[0].fill_(-1.0)
+    torch.cuda.synchronize()
+    p.flags = _lib.FLAG_GRAPH
+    eng.run_batch_dev(xd.data_ptr(), offs, E, sig, p, avgs[1].data_ptr(), res.data_ptr(),
+                      nm.data_ptr(), nmode.data_ptr(), nsift.data_ptr())
+    torch.cuda.synchronize()
+    assert np.all(avgs[0].cpu().numpy() == -1.0)
+    assert np.array_equal(avgs[1].cpu().numpy(), ref["avg"])
+
+
+def test_graph_replay_after_staging_buffers_grow(eng):
+    """Host-buffer path: an eager call with more IMFs grows (reallocates) the
+    context's output staging between two graph calls of the same batch; the
+    second graph call must not write the freed buffer."""
+    x, offs, sig = _batch(7, [4000, 5000])
+    E = 4
+    ref = eng.run_batch(x, offs, E, sig, _params(1, max_imf=3))
+    g1 = eng.run_batch(x, offs, E, sig, _params(1, max_imf=3), graph=True)
+    cap1, _ = eng.graph_stats()
+    big = eng.run_batch(x, offs, E, sig, _params(1, max_imf=12))
+    assert np.all(big["n_modes"] > 0)
+    print("case: graph after staging growth", flush=True)
+    g2 = eng.run_batch(x, offs, E, sig, _params(1, max_imf=3), graph=True)
+    cap2, _ = eng.graph_stats()
+    assert cap2 == cap1 + 1
+    for g in (g1, g2):
+        assert np.array_equal(g["avg"], ref["avg"])
+        assert np.array_equal(g["resid"], ref["resid"])
+        assert np.array_equal(g["nsift"], ref["nsift"])
+
+
+def test_async_calls_on_two_streams_are_ordered(eng):
+    """Two asynchronous calls of one context on different streams share the
+    context's scratch: the second waits for the first, both results exact."""
+    import torch
+    from eemd4cuda_amd import _lib
+    dev = torch.device("cuda", 0)
+    jobs = [_batch(8, [20000, 15000]), _batch(9, [9000, 30000, 4000])]
+    E, mi = 6, 6
+    refs = [eng.run_batch(x, o, E, s, _params(1, max_imf=mi)) for x, o, s in jobs]
+    streams = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
+    bufs = []
+    for (x, offs, sig), st in zip(jobs, streams):
+        n, nsig = len(x), len(offs) - 1
+        b = dict(x=torch.from_numpy(x).to(dev),
+                 avg=torch.zeros(mi * n, dtype=torch.float64, device=dev),
+                 res=torch.zeros(n, dtype=torch.float64, device=dev),
+                 nm=torch.zeros(nsig, dtype=torch.int32, device=dev),
+                 nmode=torch.zeros(nsig * E, dtype=torch.int32, device=dev),
+                 nsift=torch.zeros(nsig * E, dtype=torch.int32, device=dev))
+        bufs.append(b)
+    torch.cuda.synchronize()
+    print("case: two async calls on two streams", flush=True)
+    for (x, offs, sig), st, b in zip(jobs, streams, bufs):
+        p = _params(1, max_imf=mi)
+        eng.run_batch_dev(b["x"].data_ptr(), offs, E, sig, p, b["avg"].data_ptr(),
+                          b["res"].data_ptr(), b["nm"].data_ptr(), b["nmode"].data_ptr(),
+                          b["nsift"].data_ptr(), stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    for ref, b in zip(refs, bufs):
+        assert np.array_equal(b["avg"].cpu().numpy(), ref["avg"])
+        assert np.array_equal(b["nsift"].cpu().numpy(), ref["nsift"].ravel())
+
+
+def test_autotune_fp64_learned_ends(eng):
+    """FP64 with learned ends: the tuner skips the workgroup size that cannot
+    hold the model in LDS and returns a valid choice."""
+    import torch
+    from eemd4cuda_amd import learned
+    x, offs, sig = _batch(10, [6000, 12000])
+    E, mi = 4, 4
+    learned.install(learned.load(learned.default_model_path()), eng)
+    xd = torch.from_numpy(x).to(torch.device("cuda", 0))
+    p = _params(1, B=512, max_imf=mi)
+    from eemd4cuda_amd import _lib
+    p.end_mode = _lib.END_MODES["learned"]
+    print("case: autotune fp64 learned", flush=True)
+    best, ms = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
+    assert best.threads in (256, 512) and ms > 0
+    p.end_mode = _lib.END_MODES["mirror"]
