@@ -334,7 +334,7 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
     if (!p->fp64) P.rows = std::max<int64_t>(P.rows, 3 * 64 * WAVE * 4 / (2 * 4 * WAVE));
     P.cols = P.threads;
   }
-  P.stride = make_layout(tb, P.npad, nw, P.cap, P.rows, P.cols).total;
+  P.stride = make_layout(tb, P.npad, nw, P.cap, P.rows, P.cols, p->tau > 0.0 ? 1 : 0).total;
   return EEMD_OK;
 }
 

@@ -62,13 +62,26 @@ struct ScratchLayout {
   int64_t km;         // knot second derivatives, same slots, T each
   int64_t wcx;        // window Thomas forward sweep {c', x'} [rows][cols]
                       // (double2 in FP64 mode, float2 in FP32 mode)
+  int64_t sd;         // SD stop terms (tau > 0 only): per sample (h - h')^2 and
+                      // h^2 [2][npad] double, then their leaf sums [2][sd_leaves]
   int64_t total;
 };
+
+// numpy's pairwise summation (np.sum of a contiguous float64 array): blocks
+// of PW_BLOCK elements added in sequence; inside a block, ranges longer than
+// PW_LEAF split in two; a range of at most PW_LEAF elements is a leaf.
+constexpr int PW_BLOCK = 8192;
+constexpr int PW_LEAF = 128;
+constexpr int PW_MAXPART = 130;  // leaves of a partial block (each >= 64 elements)
+
+// Leaf-sum slots of the SD region for a padded length npad.
+__host__ __device__ inline int64_t sd_leaves(int64_t npad) { return npad / PW_LEAF + PW_MAXPART + 2; }
 
 __host__ __device__ inline int64_t align256(int64_t v) { return (v + 255) & ~int64_t(255); }
 
 __host__ __device__ inline ScratchLayout make_layout(int tbytes, int64_t npad, int nseg,
-                                                     int64_t cap, int64_t rows, int64_t cols) {
+                                                     int64_t cap, int64_t rows, int64_t cols,
+                                                     int sd) {
   ScratchLayout L;
   int64_t o = 0;
   L.h = o;    o = align256(o + npad * tbytes);
@@ -77,6 +90,7 @@ __host__ __device__ inline ScratchLayout make_layout(int tbytes, int64_t npad, i
   L.kty = o;  o = align256(o + 4 * (nseg * cap + VSLOTS) * 2 * tbytes);
   L.km = o;   o = align256(o + 4 * (nseg * cap + VSLOTS) * tbytes);
   L.wcx = o;  o = align256(o + rows * cols * 2 * tbytes);
+  L.sd = o;   if (sd) o = align256(o + 8 * (2 * npad + 2 * sd_leaves(npad)));
   L.total = o;
   return L;
 }

@@ -57,11 +57,17 @@ struct Shared {
   int32_t p[2], nl[2], nr[2], P[2];  // real, left / right virtual, total knots
   T cval[2];                         // constant envelope when p < 2
   T vt[2][2 * MAX_VIRT];             // positions of the virtual knots (interval search)
-  double red[2][NW];
   int32_t acc_cnt[MAX_IMF_CAP];
   int32_t pm[MAX_IMF_CAP];
   int32_t task;
   double sd;
+  // the SD stop's pairwise sums: leaves (offset, length) of the member's
+  // partial last block in summation order, and a stack for combining them
+  int32_t pw_off[PW_MAXPART], pw_len[PW_MAXPART];
+  int32_t pw_n;
+  int32_t pws_l[16], pws_n2[16], pws_st[16];
+  double pws_a[16], pws_b[16];
+  double pw_part[2];
   // per-wave staging of one envelope's new knots of a chunk (at most 4 per
   // lane: strict extrema are >= 2 samples apart), copied out coalesced
   typename std::conditional<std::is_same<T, float>::value, float2, double2>::type
@@ -238,6 +244,7 @@ struct Mem {
   int64_t vbase;    // slot of virtual knot 0 (= NW*cap)
   void* wcx;        // window forward sweep {c', x'} [rows][cols] (T2)
   int64_t rows, cols;
+  double* sdb;      // SD terms: (h - h')^2 [npad], h^2 [npad], leaf sums (tau > 0)
 };
 
 template <typename T>
@@ -801,8 +808,7 @@ __device__ __forceinline__ void detect_compact(Shared<T, NW>& S, const Mem<T>& M
 template <typename T, int NW>
 __device__ __forceinline__ void pass_epilogue(Shared<T, NW>& S, int nxt, int w,
                                               const int (&run)[2],
-                                              const unsigned long long (&hs)[2], int trace,
-                                              bool want_sd, double num, double den) {
+                                              const unsigned long long (&hs)[2], int trace) {
   const int ln = lane_id();
   if (trace) {
     const unsigned long long a = wave_sum_u64(hs[0]);
@@ -812,17 +818,11 @@ __device__ __forceinline__ void pass_epilogue(Shared<T, NW>& S, int nxt, int w,
       atomicAdd(&S.hsh[nxt][1], b);
     }
   }
-  if (want_sd) {
-    num = wave_sum(num);
-    den = wave_sum(den);
-  }
   if (ln == 0) {
     S.cnt[nxt][0][w] = run[0];
     S.cnt[nxt][1][w] = run[1];
     S.beg[nxt][0][w] = SEG_FRONT;
     S.beg[nxt][1][w] = SEG_FRONT;
-    S.red[0][w] = num;
-    S.red[1][w] = den;
   }
 }
 
@@ -843,7 +843,6 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
   if (seg_hi > n) seg_hi = n;
   int run[2] = {0, 0};
   unsigned long long hs[2] = {0ull, 0ull};
-  double num = 0.0, den = 0.0;
   T carry1 = (T)0, carry2 = (T)0;
   constexpr bool FIXED = std::is_same<T, double>::value;
   const bool want_sd = !LAST && io.want_sd;
@@ -922,13 +921,17 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
       T v[RPL];
       if constexpr (!LAST) {
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-          v[k] = hv[k] - (T)0.5 * msum[k];
-          if (want_sd && i0 + k < ni) {
-            const T dd = hv[k] - v[k];
-            num += (double)(dd * dd);
-            den += (double)(hv[k] * hv[k]);
-          }
+        for (int k = 0; k < RPL; ++k) v[k] = hv[k] - (T)0.5 * msum[k];
+        if (want_sd) {
+          // the terms of the SD sums, summed afterwards in numpy's order
+          const int64_t np = (int64_t)M.bstride * 8;
+#pragma unroll
+          for (int k = 0; k < RPL; ++k)
+            if (i0 + k < ni) {
+              const T dd = hv[k] - v[k];
+              M.sdb[i0 + k] = (double)(dd * dd);
+              M.sdb[np + i0 + k] = (double)(hv[k] * hv[k]);
+            }
         }
         store8(M.h + (uint32_t)i0, v);
       } else {
@@ -968,7 +971,7 @@ __device__ __forceinline__ void wave_sift_pipe(Shared<T, NW>& S, const Mem<T>& M
       fb1 = gb1;
     }
   }
-  pass_epilogue(S, nxt, w, run, hs, io.trace, want_sd, num, den);
+  pass_epilogue(S, nxt, w, run, hs, io.trace);
 }
 
 // INIT / FINAL pass: one wave streams its segment, produces the new residual
@@ -1028,7 +1031,183 @@ __device__ __forceinline__ void wave_pass(Shared<T, NW>& S, const Mem<T>& M, con
     detect_compact<T, NW>(S, M, nxt, cb, seg_lo, v, carry1, carry2, run, hs, io.trace,
                           (int64_t)w * M.cap + SEG_FRONT);
   }
-  pass_epilogue(S, nxt, w, run, hs, io.trace, false, 0.0, 0.0);
+  pass_epilogue(S, nxt, w, run, hs, io.trace);
+}
+
+// ------------------------------------------------------------------------
+// SD stop test in numpy's summation order (emd_ref.py:141-142 sums with
+// np.sum): the pass stores the per-sample terms, these routines add them
+// the way numpy's pairwise summation does.
+// ------------------------------------------------------------------------
+// Sum of one leaf of n <= PW_LEAF elements: in order when n < 8; otherwise
+// eight running sums of stride-8 elements, combined as
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the remaining elements in order.
+__device__ __forceinline__ double pw_leaf(const double* a, int n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  double r0 = a[0], r1 = a[1], r2 = a[2], r3 = a[3], r4 = a[4], r5 = a[5], r6 = a[6], r7 = a[7];
+  int i = 8;
+  for (; i < n - (n % 8); i += 8) {
+    r0 += a[i];
+    r1 += a[i + 1];
+    r2 += a[i + 2];
+    r3 += a[i + 3];
+    r4 += a[i + 4];
+    r5 += a[i + 5];
+    r6 += a[i + 6];
+    r7 += a[i + 7];
+  }
+  double res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+  for (; i < n; ++i) res += a[i];
+  return res;
+}
+
+// Split point of a range of l > PW_LEAF elements: half, rounded down to a
+// multiple of 8.
+__device__ __forceinline__ int pw_split(int l) {
+  const int n2 = l / 2;
+  return n2 - n2 % 8;
+}
+
+// Thread 0, once per member: the leaves of the partial last block (n mod
+// PW_BLOCK elements) in summation order (depth-first, left half first).
+template <typename T, int NW>
+__device__ void pw_enumerate(Shared<T, NW>& S, int64_t n) {
+  const int m = (int)(n % PW_BLOCK);
+  int cnt = 0;
+  if (m > 0) {
+    int sp = 0;
+    S.pws_l[0] = m;
+    S.pws_n2[0] = 0;  // offset of the range
+    sp = 1;
+    while (sp > 0) {
+      --sp;
+      const int l = S.pws_l[sp], o = S.pws_n2[sp];
+      if (l <= PW_LEAF) {
+        S.pw_off[cnt] = o;
+        S.pw_len[cnt] = l;
+        ++cnt;
+        continue;
+      }
+      const int n2 = pw_split(l);
+      S.pws_l[sp] = l - n2;  // right half, taken after the left one
+      S.pws_n2[sp] = o + n2;
+      ++sp;
+      S.pws_l[sp] = n2;
+      S.pws_n2[sp] = o;
+      ++sp;
+    }
+  }
+  S.pw_n = cnt;
+}
+
+// All threads, after a pass that stored the SD terms: S.sd = num / den (0
+// when den is 0) with num and den summed in numpy's order.
+template <typename T, int NW, int NT>
+__device__ void sd_reduce(Shared<T, NW>& S, const Mem<T>& M) {
+  const int tid = (int)threadIdx.x;
+  const int64_t np = (int64_t)M.bstride * 8;
+  const double* qn = M.sdb;
+  const double* qd = M.sdb + np;
+  double* ln = M.sdb + 2 * np;
+  double* ld = ln + sd_leaves(np);
+  const int64_t nbf = M.n / PW_BLOCK;           // full blocks
+  const int64_t nfl = nbf * (PW_BLOCK / PW_LEAF);  // their leaves
+  const int64_t nl = nfl + S.pw_n;
+  for (int64_t j = tid; j < nl; j += NT) {
+    int64_t o;
+    int len;
+    if (j < nfl) {
+      o = j * PW_LEAF;
+      len = PW_LEAF;
+    } else {
+      o = nbf * PW_BLOCK + S.pw_off[j - nfl];
+      len = S.pw_len[j - nfl];
+    }
+    ln[j] = pw_leaf(qn + o, len);
+    ld[j] = pw_leaf(qd + o, len);
+  }
+  __syncthreads();
+  // full blocks: a balanced tree over their 64 leaves, in place
+  for (int64_t b = tid; b < nbf; b += NT) {
+    double* vn = ln + b * (PW_BLOCK / PW_LEAF);
+    double* vd = ld + b * (PW_BLOCK / PW_LEAF);
+    for (int st = 1; st < PW_BLOCK / PW_LEAF; st <<= 1)
+      for (int k = 0; k < PW_BLOCK / PW_LEAF; k += 2 * st) {
+        vn[k] = vn[k] + vn[k + st];
+        vd[k] = vd[k] + vd[k + st];
+      }
+  }
+  // the partial block: its split tree over its leaves, depth first
+  if (tid == NT - 1) {
+    double rn = 0.0, rd = 0.0;
+    const int m = (int)(M.n % PW_BLOCK);
+    if (m > 0) {
+      const double* vn = ln + nfl;
+      const double* vd = ld + nfl;
+      int li = 0, sp = 1;
+      S.pws_l[0] = m;
+      S.pws_st[0] = 0;
+      for (;;) {
+        const int f = sp - 1;
+        bool deliver = false;
+        if (S.pws_st[f] == 0) {
+          const int l = S.pws_l[f];
+          if (l <= PW_LEAF) {
+            rn = vn[li];
+            rd = vd[li];
+            ++li;
+            --sp;
+            deliver = true;
+          } else {
+            const int n2 = pw_split(l);
+            S.pws_n2[f] = n2;
+            S.pws_st[f] = 1;
+            S.pws_l[sp] = n2;
+            S.pws_st[sp] = 0;
+            ++sp;
+          }
+        }
+        while (deliver) {
+          if (sp == 0) break;
+          const int p = sp - 1;
+          if (S.pws_st[p] == 1) {
+            // left half done: keep it, sum the right half next
+            S.pws_a[p] = rn;
+            S.pws_b[p] = rd;
+            S.pws_st[p] = 2;
+            S.pws_l[sp] = S.pws_l[p] - S.pws_n2[p];
+            S.pws_st[sp] = 0;
+            ++sp;
+            deliver = false;
+          } else {
+            rn = S.pws_a[p] + rn;
+            rd = S.pws_b[p] + rd;
+            --sp;
+          }
+        }
+        if (sp == 0) break;
+      }
+    }
+    S.pw_part[0] = rn;
+    S.pw_part[1] = rd;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double nu = 0.0, de = 0.0;
+    for (int64_t b = 0; b < nbf; ++b) {
+      nu = nu + ln[b * (PW_BLOCK / PW_LEAF)];
+      de = de + ld[b * (PW_BLOCK / PW_LEAF)];
+    }
+    if (M.n % PW_BLOCK) {
+      nu = nu + S.pw_part[0];
+      de = de + S.pw_part[1];
+    }
+    S.sd = (de > 0.0) ? nu / de : 0.0;
+  }
 }
 
 // Positions seg_lo-1 and seg_lo of each segment need values of the previous
@@ -1783,7 +1962,8 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   const int tid = (int)threadIdx.x;
 
   char* scr = A.scratch + (int64_t)blockIdx.x * A.scratch_stride;
-  const ScratchLayout LY = make_layout((int)sizeof(T), A.npad, NW, A.cap, A.rows, A.cols);
+  const ScratchLayout LY = make_layout((int)sizeof(T), A.npad, NW, A.cap, A.rows, A.cols,
+                                       A.tau > 0.0 ? 1 : 0);
   Mem<T> M;
   M.cap = A.cap;
   M.h = reinterpret_cast<T*>(scr + LY.h);
@@ -1795,6 +1975,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
   M.rstride = (int64_t)NW * A.cap + VSLOTS;
   M.vbase = (int64_t)NW * A.cap;
   M.wcx = reinterpret_cast<void*>(scr + LY.wcx);
+  M.sdb = reinterpret_cast<double*>(scr + LY.sd);
   M.rows = A.rows;
   M.cols = A.cols;
 
@@ -1856,6 +2037,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
       if (tid == 0) {
         S.hsh[0][0] = S.hsh[0][1] = 0ull;
         S.hsh[1][0] = S.hsh[1][1] = 0ull;
+        if (A.tau > 0.0) pw_enumerate(S, M.n);
       }
       __syncthreads();
       PHASE(3);
@@ -1918,15 +2100,8 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           PHASE(1);
           fixup(S, M, last ? M.r : M.h, nxt, trace);
           __syncthreads();
-          if (tid == 0) {
-            prefix(S, nxt);
-            double nu = 0.0, de = 0.0;
-            for (int w = 0; w < NW; ++w) {
-              nu += S.red[0][w];
-              de += S.red[1][w];
-            }
-            S.sd = (de > 0.0) ? nu / de : 0.0;
-          }
+          if (tid == 0) prefix(S, nxt);
+          if (!last && io.want_sd) sd_reduce<T, NW, NT>(S, M);
           __syncthreads();
           PHASE(2);
           ++ns;

@@ -130,7 +130,7 @@ def test_uneven_batch_fp64_matches_oracle(eng, uneven):
     sig = np.array([0.2 * np.std(v) for v in xs])
     E, mi = 100, 8
     print("case: uneven 512 signals E=100 fp64 one batched call, trace", flush=True)
-    out = eng.run_batch(np.concatenate(xs), offs, E, sig, params(max_imf=mi, mpt=1, seed=2),
+    out = eng.run_batch(np.concatenate(xs), offs, E, sig, params(max_imf=mi, mpt=0, seed=2),
                         trace=True)
     xcat = np.concatenate(xs)
     ms = means(out, lens, mi)
@@ -182,8 +182,9 @@ def test_uneven_batch_fp32_error_per_imf(eng, uneven):
         for m in range(min(len(m32[s]), len(m64[s]))):
             worst[m] = max(worst[m], float(np.linalg.norm(m32[s][m] - m64[s][m]) / xn))
     print("fp32 worst-signal L2 per IMF vs fp64 / |x|:", ["%.2e" % e for e in worst])
-    # measured 3.8e-4 .. 2.8e-3 of |x| for IMFs 1-4
-    assert np.all(worst[:4] <= 1e-2)
+    # measured 2.4e-4 6.0e-4 9.5e-4 2.8e-3 of |x| for IMFs 1-4 (later levels: members
+    # disagree on which levels exist, 2.5e-2 .. 0.55)
+    assert np.all(worst[:4] <= np.array([1e-3, 2e-3, 3e-3, 1e-2])), worst
 
 
 def test_chirp_learned_fp64_matches_oracle(eng):

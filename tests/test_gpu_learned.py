@@ -66,7 +66,9 @@ def test_mlp_gradient_matches_reference(eng):
         ew = np.linalg.norm(gw - rw) / np.linalg.norm(rw)
         eb = np.linalg.norm(gb - rb) / np.linalg.norm(rb)
         print("layer", l, "rel err W %.2e b %.2e" % (ew, eb))
-        assert ew <= 1e-2 and eb <= 1e-2, (l, ew, eb)
+        # measured at most 3.1e-4 (weights) and 9.4e-5 (biases): FP32 accumulation of
+        # BF16 products in MFMA order against the float64 reference of the same products
+        assert ew <= 1e-3 and eb <= 1e-3, (l, ew, eb)
     # the parameters did not move
     assert np.array_equal(net.params(), p)
 

@@ -195,10 +195,15 @@ def test_fp32_ensemble_error(eng, golden):
     out = run1(eng, x, z, sigma, fp64=0, B=1024, S=14)
     errs = [rel_l2(out["avg2"][m], g["avg"][m]) for m in range(min(len(g["avg"]), len(out["avg2"])))]
     print("fp32 per-IMF rel L2 vs reference:", ["%.2e" % e for e in errs])
-    # noise-dominated IMFs are stable; slow IMFs carry the flips (a knot moved
-    # by one sample in a member changes its envelope locally by O(1/spacing))
-    assert errs[0] <= 2e-2 and errs[1] <= 2e-2
-    assert max(errs[:6]) <= 0.25
+    # measured 3.2e-3 4.9e-3 3.1e-4 9.2e-4 5.2e-4 2.2e-2 0.12 0.096: slow IMFs carry
+    # the flips (a knot moved by one sample in a member changes its envelope
+    # locally by O(1/spacing)); bounds just above the measured values
+    bound = [1e-2, 1.5e-2, 1e-3, 3e-3, 2e-3, 6e-2, 0.3, 0.3]
+    for m, b in enumerate(bound[:len(errs)]):
+        assert errs[m] <= b, (m, errs[m])
+    # every member's IMF count equals FP64's on this signal
+    o64 = run1(eng, x, z, sigma, B=1024, S=30)
+    assert np.array_equal(out["nmode"], o64["nmode"])
 
 
 def test_determinism(eng):
@@ -252,3 +257,31 @@ def test_members_per_task_only_reassociates(eng):
     b = run1(eng, x, None, 0.1, E=16, B=0, mpt=5)
     assert np.array_equal(a["mimf"], b["mimf"])
     assert np.max(np.abs(a["avg"] - b["avg"])) <= 1e-13
+
+
+def _sd_case(N):
+    rng = np.random.default_rng(N)
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 50.0) + 0.5 * np.sin(2 * np.pi * t / 7.3) + 0.1 * rng.standard_normal(N)
+    # h after sifts 1 and 2 of the first IMF (tau = 0: exactly that many sifts)
+    h1 = orc.emd_1d(x, tau=0.0, max_sifts=1, max_imf=1, end="mirror")[0][0]
+    h2 = orc.emd_1d(x, tau=0.0, max_sifts=2, max_imf=1, end="mirror")[0][0]
+    sd = float(np.sum((h1 - h2) ** 2) / np.sum(h1 ** 2))   # emd_ref.py:141-143
+    return x, sd
+
+
+@pytest.mark.parametrize("N", [1000, 8192, 20000, 40961])
+def test_sd_stop_decision_at_one_ulp(eng, N):
+    """tau set to the reference's SD of the second sift, and one ulp either
+    side: the stop decision hinges on the last bit of the SD ratio, so the
+    sums must be added in np.sum's pairwise order (blocks of 8192, pairwise
+    inside, leaves of 128 with eight running sums)."""
+    x, sd = _sd_case(N)
+    for tau, stops in ((sd, False), (np.nextafter(sd, np.inf), True), (np.nextafter(sd, -np.inf), False)):
+        print("case: SD stop N=%d tau=%r" % (N, tau), flush=True)
+        out = run1(eng, x, np.zeros((1, N)), 0.0, end="mirror", tau=tau, max_sifts=50, max_imf=1,
+                   B=0)
+        _, _, st = orc.emd_1d(x, tau=tau, max_sifts=50, max_imf=1, end="mirror")
+        ns = int(out["per_mode"][0, 0, 0])
+        assert ns == st["per_mode_sifts"][0]
+        assert (ns == 2) == stops, (tau, ns)

@@ -278,3 +278,21 @@ def test_pairwise_sum_is_numpy_sum():
     for n in (1, 7, 8, 9, 127, 128, 129, 1000, 4097, 100003):
         a = rng.standard_normal(n) * 10.0 ** rng.uniform(-8, 8, n)
         assert orc.pairwise_sum(a) == np.sum(a)
+
+
+@pytest.mark.parametrize("N", [1000, 20000])
+def test_oracle_sd_stop_decision_at_one_ulp_equals_numpy(N):
+    """The oracle's SD stop, at tau one ulp either side of the SD of the
+    second sift, decides as the numpy restatement of emd_ref (np.sum)."""
+    from oracle import emd_np
+    rng = np.random.default_rng(N)
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 50.0) + 0.5 * np.sin(2 * np.pi * t / 7.3) + 0.1 * rng.standard_normal(N)
+    h1 = orc.emd_1d(x, tau=0.0, max_sifts=1, max_imf=1, end="mirror")[0][0]
+    h2 = orc.emd_1d(x, tau=0.0, max_sifts=2, max_imf=1, end="mirror")[0][0]
+    sd = float(np.sum((h1 - h2) ** 2) / np.sum(h1 ** 2))
+    for tau in (sd, np.nextafter(sd, np.inf), np.nextafter(sd, -np.inf)):
+        _, _, st = orc.emd_1d(x, tau=tau, max_sifts=50, max_imf=1, end="mirror")
+        _, _, sn = emd_np.emd_1d(x, tau=tau, max_sifts=50, max_imf=1, end="mirror")
+        assert st["per_mode_sifts"] == sn["per_mode_sifts"]
+    assert st["per_mode_sifts"][0] > 2
