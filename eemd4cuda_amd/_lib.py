@@ -65,6 +65,7 @@ def load(path: Path | None = None):
     run_args = [vp, vp, vp, i32, i32, vp, vp, C.POINTER(Params), vp, vp, vp, vp, vp, vp, vp, vp]
     L.eemd_run_batch.argtypes = run_args
     L.eemd_run_batch_dev.argtypes = run_args + [vp]
+    L.eemd_run_batch_dev_noalloc.argtypes = run_args + [vp]
     L.eemd_noise.argtypes = [vp, u64, i32, i32, i32, i64, vp]
     L.eemd_autotune.argtypes = [vp, vp, vp, i32, i32, vp, vp, C.POINTER(Params), i32,
                                 C.POINTER(Params), C.POINTER(dbl)]
@@ -74,6 +75,7 @@ def load(path: Path | None = None):
     L.eemd_graph_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.eemd_mem_info.argtypes = [vp, C.POINTER(u64), C.POINTER(u64)]
     for name in ("eemd_ctx_create", "eemd_ctx_destroy", "eemd_run_batch", "eemd_run_batch_dev",
+                 "eemd_run_batch_dev_noalloc",
                  "eemd_noise", "eemd_last_timing", "eemd_last_phases", "eemd_autotune",
                  "eemd_graph_stats", "eemd_mem_info"):
         getattr(L, name).restype = C.c_int
@@ -178,21 +180,25 @@ class Engine:
     def run_batch_dev(self, x_ptr: int, offsets: np.ndarray, E: int, sigmas: np.ndarray,
                       prm: Params, avg_ptr: int, resid_ptr: int, n_modes_ptr: int,
                       nmode_ptr: int, nsift_ptr: int, per_mode_ptr: int | None = None,
-                      noise_ptr: int | None = None, stream: int | None = None):
-        """Device-buffer variant (eemd_run_batch_dev): pointers are device addresses."""
+                      noise_ptr: int | None = None, stream: int | None = None,
+                      noalloc: bool = False):
+        """Device-buffer variant (eemd_run_batch_dev): pointers are device
+        addresses.  noalloc=True calls eemd_run_batch_dev_noalloc, which takes
+        its scratch from a static device array instead of allocating."""
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         sigmas = np.ascontiguousarray(sigmas, dtype=np.float64)
         nsig = len(offsets) - 1
         prm.flags = prm.flags & (FLAG_PHASES | FLAG_GRAPH)
         vp = C.c_void_p
+        fn = self.lib.eemd_run_batch_dev_noalloc if noalloc else self.lib.eemd_run_batch_dev
         with _lock:
-            rc = self.lib.eemd_run_batch_dev(
+            rc = fn(
                 self.h, vp(x_ptr), _ptr(offsets), nsig, int(E), _ptr(sigmas),
                 vp(noise_ptr) if noise_ptr else None, C.byref(prm), vp(avg_ptr), vp(resid_ptr),
                 vp(n_modes_ptr), vp(nmode_ptr), vp(nsift_ptr),
                 vp(per_mode_ptr) if per_mode_ptr else None, None, None,
                 vp(stream) if stream else None)
-        self._check(rc, "eemd_run_batch_dev")
+        self._check(rc, "eemd_run_batch_dev_noalloc" if noalloc else "eemd_run_batch_dev")
 
     def autotune(self, x_ptr: int, offsets: np.ndarray, E: int, sigmas: np.ndarray, prm: Params,
                  noise_ptr: int | None = None, reps: int = 2):

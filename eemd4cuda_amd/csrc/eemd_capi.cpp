@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -34,7 +35,32 @@ hipError_t launch_noise(uint64_t seed, uint32_t sig, uint32_t m0, int E, int64_t
 
 using namespace eemd;
 
+// Device memory of the allocation-free entry point (eemd_run_batch_dev_noalloc):
+// every GPU the library runs on holds its own copy until the process ends.
+// Host code never uses this name as an address: the address on each device
+// comes from hipGetSymbolAddress at context creation (noalloc_addr below).
+static __device__ __align__(256) unsigned char scratch[EEMD_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+// Per-device address of `scratch`, looked up once per device with that
+// device current.
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[64] = {};
+hipError_t g_noalloc_err[64] = {};
+bool g_noalloc_done[64] = {};
+
+void noalloc_lookup(int device) {
+  if (device < 0 || device >= 64) return;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_done[device]) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) (void)hipGetLastError();
+  g_noalloc_ptr[device] = (e == hipSuccess) ? ptr : nullptr;
+  g_noalloc_err[device] = e;
+  g_noalloc_done[device] = true;
+}
 
 // A device buffer that only grows.
 struct DevBuf {
@@ -338,20 +364,47 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   return EEMD_OK;
 }
 
-hipError_t upload_desc(const Plan& P, DescBufs& D, hipStream_t st) {
+// Device buffers one call's kernels use besides the caller's inputs and
+// outputs: the context's grow-only buffers, or carve-outs of the static
+// allocation-free scratch.
+struct DevPtrs {
+  char* scratch = nullptr;  // per-workgroup slabs
+  double* acc = nullptr;
+  int32_t* acc_cnt = nullptr;
+  int32_t* queue = nullptr;
+  uint64_t* phase = nullptr;
+  SignalDesc* sigd = nullptr;
+  TaskDesc* taskd = nullptr;
+  int32_t* order = nullptr;
+  int64_t* sig_start = nullptr;
+};
+
+hipError_t ensure_desc(const Plan& P, DescBufs& D) {
   const size_t nt = P.task.size();
   hipError_t e;
   if ((e = D.sigd.ensure(sizeof(SignalDesc) * P.sig.size())) != hipSuccess) return e;
   if ((e = D.taskd.ensure(sizeof(TaskDesc) * nt)) != hipSuccess) return e;
   if ((e = D.order.ensure(sizeof(int32_t) * nt)) != hipSuccess) return e;
-  if ((e = D.sig_start.ensure(sizeof(int64_t) * P.start.size())) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(D.sigd.p, P.sig.data(), sizeof(SignalDesc) * P.sig.size(),
+  return D.sig_start.ensure(sizeof(int64_t) * P.start.size());
+}
+
+void desc_ptrs(const DescBufs& D, DevPtrs& Q) {
+  Q.sigd = (SignalDesc*)D.sigd.p;
+  Q.taskd = (TaskDesc*)D.taskd.p;
+  Q.order = (int32_t*)D.order.p;
+  Q.sig_start = (int64_t*)D.sig_start.p;
+}
+
+hipError_t upload_desc(const Plan& P, const DevPtrs& D, hipStream_t st) {
+  const size_t nt = P.task.size();
+  hipError_t e;
+  if ((e = hipMemcpyAsync(D.sigd, P.sig.data(), sizeof(SignalDesc) * P.sig.size(),
                           hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(D.taskd.p, P.task.data(), sizeof(TaskDesc) * nt,
+  if ((e = hipMemcpyAsync(D.taskd, P.task.data(), sizeof(TaskDesc) * nt,
                           hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(D.order.p, P.order.data(), sizeof(int32_t) * nt,
+  if ((e = hipMemcpyAsync(D.order, P.order.data(), sizeof(int32_t) * nt,
                           hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-  return hipMemcpyAsync(D.sig_start.p, P.start.data(), sizeof(int64_t) * P.start.size(),
+  return hipMemcpyAsync(D.sig_start, P.start.data(), sizeof(int64_t) * P.start.size(),
                         hipMemcpyHostToDevice, st);
 }
 
@@ -359,13 +412,13 @@ hipError_t upload_desc(const Plan& P, DescBufs& D, hipStream_t st) {
 // sequence can be captured), the persistent sift kernel and the mean kernel.
 // With `events`, ev[0..2] bracket the sift and mean kernels.
 hipError_t enqueue_work(eemd_ctx* c, const Plan& P, SiftArgs A, const eemd_params* p,
-                        const DescBufs& D, const double* x, double* avg, double* resid,
+                        const DevPtrs& D, const double* x, double* avg, double* resid,
                         int32_t* n_modes, hipStream_t st, bool events) {
   hipError_t e;
-  A.sig = (const SignalDesc*)D.sigd.p;
-  A.task = (const TaskDesc*)D.taskd.p;
-  A.order = (const int32_t*)D.order.p;
-  if ((e = launch_fill_i32((int32_t*)c->queue.p, 1, 0, st)) != hipSuccess) return e;
+  A.sig = D.sigd;
+  A.task = D.taskd;
+  A.order = D.order;
+  if ((e = launch_fill_i32(D.queue, 1, 0, st)) != hipSuccess) return e;
   if ((e = launch_fill_f64(avg, (int64_t)p->max_imf * P.total_n, 0.0, st)) != hipSuccess) return e;
   const int tb = p->fp64 ? 8 : 4;
   if (A.member_imfs) {
@@ -380,11 +433,9 @@ hipError_t enqueue_work(eemd_ctx* c, const Plan& P, SiftArgs A, const eemd_param
   if (events && (e = hipEventRecord(c->ev[0], st)) != hipSuccess) return e;
   if ((e = launch_sift(A, p->fp64, P.threads, P.grid, st)) != hipSuccess) return e;
   if (events && (e = hipEventRecord(c->ev[1], st)) != hipSuccess) return e;
-  if ((e = launch_mean((const SignalDesc*)D.sigd.p, (int)P.sig.size(),
-                       (const int64_t*)D.sig_start.p, (const TaskDesc*)D.taskd.p,
-                       (const double*)c->acc.p, (const int32_t*)c->acc_cnt.p, p->max_imf, x, avg,
-                       resid, n_modes, P.total_n, (p->flags & EEMD_FLAG_LEVEL_SUMS) ? 1 : 0,
-                       st)) != hipSuccess)
+  if ((e = launch_mean(D.sigd, (int)P.sig.size(), D.sig_start, D.taskd, D.acc, D.acc_cnt,
+                       p->max_imf, x, avg, resid, n_modes, P.total_n,
+                       (p->flags & EEMD_FLAG_LEVEL_SUMS) ? 1 : 0, st)) != hipSuccess)
     return e;
   if (events && (e = hipEventRecord(c->ev[2], st)) != hipSuccess) return e;
   return hipSuccess;
@@ -403,6 +454,48 @@ void key_put_vec(std::vector<uint8_t>& k, const std::vector<V>& v) {
   k.insert(k.end(), b, b + sizeof(V) * v.size());
 }
 
+// Kernel arguments of one call (everything but the descriptor pointers,
+// which enqueue_work fills in).
+SiftArgs sift_args(const Plan& P, const DevPtrs& D, const double* x, const double* noise,
+                   const eemd_params* p, int32_t* nmode, int32_t* nsift, int32_t* per_mode,
+                   void* mimf, uint64_t* trace, const void* mlp) {
+  SiftArgs A;
+  std::memset(&A, 0, sizeof(A));  // padding included: A is part of the graph key
+  A.x = x;
+  A.noise = noise;
+  A.ntask = (int32_t)P.task.size();
+  A.nsig = (int32_t)P.sig.size();
+  A.queue = D.queue;
+  A.seed = p->seed;
+  A.max_imf = p->max_imf;
+  A.max_sifts = p->max_sifts;
+  A.min_extrema = p->min_extrema;
+  A.end_mode = p->end_mode;
+  A.tau = p->tau;
+  A.B = p->block;
+  A.S = p->halo;
+  A.acc = D.acc;
+  A.acc_cnt = D.acc_cnt;
+  A.member_imfs = mimf;
+  A.nmode = nmode;
+  A.nsift = nsift;
+  A.per_mode = per_mode;
+  A.trace = trace;
+  A.scratch = D.scratch;
+  A.scratch_stride = P.stride;
+  A.npad = P.npad;
+  A.cap = P.cap;
+  A.rows = P.rows;
+  A.cols = P.cols;
+  A.mlp = (p->end_mode == EEMD_END_LEARNED) ? mlp : nullptr;
+  A.nslots = P.grid;
+  A.phase = (p->flags & EEMD_FLAG_PHASES) ? D.phase : nullptr;
+  A.wave_windows = P.wave_windows;
+  A.member0 = p->member0;
+  A.signal0 = p->signal0;
+  return A;
+}
+
 int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
              const eemd_params* p, double* avg, double* resid, int32_t* n_modes, int32_t* nmode,
              int32_t* nsift, int32_t* per_mode, void* mimf, uint64_t* trace, hipStream_t st,
@@ -412,45 +505,15 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   HIPCHK(c, c->acc.ensure(sizeof(double) * (size_t)std::max<int64_t>(P.acc_doubles, 1)), "acc alloc");
   HIPCHK(c, c->acc_cnt.ensure(sizeof(int32_t) * nt * p->max_imf), "acc_cnt alloc");
   HIPCHK(c, c->queue.ensure(256), "queue alloc");
-
-  SiftArgs A;
-  std::memset(&A, 0, sizeof(A));  // padding included: A is part of the graph key
-  A.x = x;
-  A.noise = noise;
-  A.ntask = (int32_t)nt;
-  A.nsig = (int32_t)P.sig.size();
-  A.queue = (int32_t*)c->queue.p;
-  A.seed = p->seed;
-  A.max_imf = p->max_imf;
-  A.max_sifts = p->max_sifts;
-  A.min_extrema = p->min_extrema;
-  A.end_mode = p->end_mode;
-  A.tau = p->tau;
-  A.B = p->block;
-  A.S = p->halo;
-  A.acc = (double*)c->acc.p;
-  A.acc_cnt = (int32_t*)c->acc_cnt.p;
-  A.member_imfs = mimf;
-  A.nmode = nmode;
-  A.nsift = nsift;
-  A.per_mode = per_mode;
-  A.trace = trace;
-  A.scratch = (char*)c->scratch.p;
-  A.scratch_stride = P.stride;
-  A.npad = P.npad;
-  A.cap = P.cap;
-  A.rows = P.rows;
-  A.cols = P.cols;
-  A.mlp = (p->end_mode == EEMD_END_LEARNED) ? c->mlpw.p : nullptr;
-  A.nslots = P.grid;
-  A.phase = nullptr;
-  A.wave_windows = P.wave_windows;
-  A.member0 = p->member0;
-  A.signal0 = p->signal0;
-  if (p->flags & EEMD_FLAG_PHASES) {
+  if (p->flags & EEMD_FLAG_PHASES)
     HIPCHK(c, c->phase.ensure(sizeof(uint64_t) * 4 * P.grid), "phase alloc");
-    A.phase = (uint64_t*)c->phase.p;
-  }
+  DevPtrs D;
+  D.scratch = (char*)c->scratch.p;
+  D.acc = (double*)c->acc.p;
+  D.acc_cnt = (int32_t*)c->acc_cnt.p;
+  D.queue = (int32_t*)c->queue.p;
+  D.phase = (uint64_t*)c->phase.p;
+  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, per_mode, mimf, trace, c->mlpw.p);
   c->last_phase_grid = (p->flags & EEMD_FLAG_PHASES) ? P.grid : 0;
   // order this call after the previous one, whatever stream it used
   if (c->done_recorded) HIPCHK(c, hipStreamWaitEvent(st, c->done, 0), "stream wait");
@@ -479,10 +542,13 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
       G.reset();
       hipStream_t cs = c->stream;
       if (st != cs) HIPCHK(c, hipStreamSynchronize(st), "stream sync before capture");
-      HIPCHK(c, upload_desc(P, G.desc, cs), "descriptor upload");
+      HIPCHK(c, ensure_desc(P, G.desc), "descriptor alloc");
+      DevPtrs DG = D;
+      desc_ptrs(G.desc, DG);
+      HIPCHK(c, upload_desc(P, DG, cs), "descriptor upload");
       HIPCHK(c, hipStreamSynchronize(cs), "descriptor upload");
       HIPCHK(c, hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
-      hipError_t e = enqueue_work(c, P, A, p, G.desc, x, avg, resid, n_modes, cs, false);
+      hipError_t e = enqueue_work(c, P, A, p, DG, x, avg, resid, n_modes, cs, false);
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cs, &g);
       if (e != hipSuccess) {
@@ -506,10 +572,101 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
     c->last_graph = true;
     ++c->graph_launches;
   } else {
-    HIPCHK(c, upload_desc(P, c->desc, st), "descriptor upload");
-    HIPCHK(c, enqueue_work(c, P, A, p, c->desc, x, avg, resid, n_modes, st, true), "kernel launch");
+    HIPCHK(c, ensure_desc(P, c->desc), "descriptor alloc");
+    desc_ptrs(c->desc, D);
+    HIPCHK(c, upload_desc(P, D, st), "descriptor upload");
+    HIPCHK(c, enqueue_work(c, P, A, p, D, x, avg, resid, n_modes, st, true), "kernel launch");
     c->last_graph = false;
   }
+  HIPCHK(c, hipEventRecord(c->done, st), "event");
+  c->done_recorded = true;
+  c->last_threads = P.threads;
+  c->last_grid = P.grid;
+  c->last_ntask = (int)nt;
+  c->timing_pending = !sync_timing;
+  if (sync_timing) {
+    HIPCHK(c, hipStreamSynchronize(st), "kernel execution");
+    HIPCHK(c, read_timing(c), "event timing");
+  }
+  return EEMD_OK;
+}
+
+// The allocation-free form of run_plan: every device buffer of the call
+// besides the caller's comes from this device's `scratch` array (its address
+// from noalloc_lookup).  The plan's grid shrinks to the slabs that fit; when
+// the accumulators of the automatic task split do not fit, every signal
+// becomes one task.  No graph capture, no buffer growth, no hipMalloc.
+int run_plan_noalloc(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E,
+                     const double* sigmas, const double* x, const double* noise,
+                     const eemd_params* p, double* avg, double* resid, int32_t* n_modes,
+                     int32_t* nmode, int32_t* nsift, int32_t* per_mode, void* mimf,
+                     uint64_t* trace, hipStream_t st, bool sync_timing) {
+  const int dev = c->device;
+  if (dev < 0 || dev >= 64 || !g_noalloc_done[dev])
+    return fail(c, EEMD_ERR_DEVICE, "allocation-free scratch: device not set up");
+  if (!g_noalloc_ptr[dev])
+    return hip_fail(c, g_noalloc_err[dev], "allocation-free scratch: hipGetSymbolAddress");
+  char* const base = static_cast<char*>(g_noalloc_ptr[dev]);
+  const int64_t cap = (int64_t)EEMD_NOALLOC_SCRATCH_BYTES;
+  Plan P;
+  int rc = make_plan(c, offsets, nsig, E, sigmas, noise, p, mimf != nullptr, P);
+  if (rc) return rc;
+  // bytes of everything but the slabs, for a plan
+  auto fixed_bytes = [&](const Plan& Q) -> int64_t {
+    const int64_t nt = (int64_t)Q.task.size();
+    int64_t b = 256;                                                   // queue
+    b += align256((int64_t)sizeof(SignalDesc) * nsig);
+    b += align256((int64_t)sizeof(TaskDesc) * nt);
+    b += align256(4 * nt);                                             // order
+    b += align256(8 * (int64_t)nsig);                                  // signal starts
+    b += align256(4 * nt * p->max_imf);                                // level counts
+    if (!per_mode) b += align256(4 * Q.units * p->max_imf);
+    if (p->flags & EEMD_FLAG_PHASES) b += align256(32 * (int64_t)Q.grid);
+    b += align256(8 * std::max<int64_t>(Q.acc_doubles, 1));            // accumulators
+    return b;
+  };
+  if (fixed_bytes(P) + P.stride > cap && p->members_per_task == 0) {
+    eemd_params q = *p;
+    q.members_per_task = E;  // one task (one accumulator block) per signal
+    rc = make_plan(c, offsets, nsig, E, sigmas, noise, &q, mimf != nullptr, P);
+    if (rc) return rc;
+  }
+  const int64_t fixed = fixed_bytes(P);
+  if (fixed + P.stride > cap) {
+    char msg[256];
+    std::snprintf(msg, sizeof msg,
+                  "batch too large for the allocation-free scratch: needs %lld bytes, "
+                  "EEMD_NOALLOC_SCRATCH_BYTES is %lld",
+                  (long long)(fixed + P.stride), (long long)cap);
+    return fail(c, EEMD_ERR_BAD_ARG, msg);
+  }
+  P.grid = (int)std::min<int64_t>(P.grid, (cap - fixed) / P.stride);
+  const size_t nt = P.task.size();
+  // carve the scratch
+  DevPtrs D;
+  int64_t o = 0;
+  auto take = [&](int64_t bytes) -> char* {
+    char* q = base + o;
+    o += align256(bytes);
+    return q;
+  };
+  D.queue = (int32_t*)take(256);
+  D.sigd = (SignalDesc*)take((int64_t)sizeof(SignalDesc) * nsig);
+  D.taskd = (TaskDesc*)take((int64_t)sizeof(TaskDesc) * nt);
+  D.order = (int32_t*)take(4 * (int64_t)nt);
+  D.sig_start = (int64_t*)take(8 * (int64_t)nsig);
+  D.acc_cnt = (int32_t*)take(4 * (int64_t)nt * p->max_imf);
+  int32_t* pm = per_mode ? per_mode : (int32_t*)take(4 * P.units * p->max_imf);
+  if (p->flags & EEMD_FLAG_PHASES) D.phase = (uint64_t*)take(32 * (int64_t)P.grid);
+  D.acc = (double*)take(8 * std::max<int64_t>(P.acc_doubles, 1));
+  D.scratch = take(P.stride * P.grid);
+  if (o > cap) return fail(c, EEMD_ERR_BAD_ARG, "allocation-free scratch overflow");
+  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, pm, mimf, trace, c->mlpw.p);
+  c->last_phase_grid = 0;  // eemd_last_phases reads the context's phase buffer
+  if (c->done_recorded) HIPCHK(c, hipStreamWaitEvent(st, c->done, 0), "stream wait");
+  HIPCHK(c, upload_desc(P, D, st), "descriptor upload");
+  HIPCHK(c, enqueue_work(c, P, A, p, D, x, avg, resid, n_modes, st, true), "kernel launch");
+  c->last_graph = false;
   HIPCHK(c, hipEventRecord(c->done, st), "event");
   c->done_recorded = true;
   c->last_threads = P.threads;
@@ -569,6 +726,8 @@ int eemd_ctx_create(int device, eemd_ctx** out) {
       return EEMD_ERR_DEVICE;
     }
   }
+  // the allocation-free scratch of this device (device current)
+  noalloc_lookup(device);
   *out = c;
   return EEMD_OK;
 }
@@ -623,6 +782,27 @@ int eemd_run_batch_dev(eemd_ctx* c, const double* x, const int64_t* offsets, int
   hipStream_t st = stream ? (hipStream_t)stream : c->stream;
   return run_plan(c, P, x, noise, p, avg, resid, n_modes, nmode, nsift, pm, mimf, trace, st,
                   stream == nullptr);
+}
+
+int eemd_run_batch_dev_noalloc(eemd_ctx* c, const double* x, const int64_t* offsets,
+                               int32_t nsig, int32_t E, const double* sigmas, const double* noise,
+                               const eemd_params* p, double* avg, double* resid, int32_t* n_modes,
+                               int32_t* nmode, int32_t* nsift, int32_t* per_mode, void* mimf,
+                               uint64_t* trace, void* stream) {
+  int rc = validate(c, offsets, nsig, E, sigmas, p);
+  if (rc) return rc;
+  if (!x || !avg || !resid || !n_modes || !nmode || !nsift)
+    return fail(c, EEMD_ERR_BAD_ARG, "required device buffer is NULL");
+  if (mimf && !(p->flags & EEMD_FLAG_MEMBER_IMFS))
+    return fail(c, EEMD_ERR_BAD_ARG, "member_imfs given without EEMD_FLAG_MEMBER_IMFS");
+  if (trace && !(p->flags & EEMD_FLAG_TRACE))
+    return fail(c, EEMD_ERR_BAD_ARG, "trace given without EEMD_FLAG_TRACE");
+  if (p->flags & EEMD_FLAG_GRAPH)
+    return fail(c, EEMD_ERR_BAD_ARG, "EEMD_FLAG_GRAPH is not available without allocation");
+  HIPCHK(c, hipSetDevice(c->device), "set device");
+  hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+  return run_plan_noalloc(c, offsets, nsig, E, sigmas, x, noise, p, avg, resid, n_modes, nmode,
+                          nsift, per_mode, mimf, trace, st, stream == nullptr);
 }
 
 int eemd_run_batch(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t nsig, int32_t E,

@@ -129,6 +129,29 @@ int eemd_run_batch_dev(eemd_ctx* ctx, const double* x, const int64_t* offsets, i
                        int32_t* n_modes, int32_t* nmode, int32_t* nsift, int32_t* per_mode_sifts,
                        void* member_imfs, uint64_t* trace, void* stream);
 
+/* Size of the allocation-free scratch: a static device array each GPU the
+ * library runs on holds until the process ends. */
+#define EEMD_NOALLOC_SCRATCH_BYTES (256ull << 20)
+
+/* eemd_run_batch_dev without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as eemd_run_batch_dev; the
+ * kernels' scratch (per-workgroup slabs, ensemble accumulators, the task
+ * queue and descriptors) comes from the static EEMD_NOALLOC_SCRATCH_BYTES
+ * array of the context's device instead of context buffers, so the launch
+ * uses as many workgroups as fit in it.  EEMD_ERR_BAD_ARG when the batch
+ * does not fit (one workgroup slab plus one accumulator block per signal)
+ * or with EEMD_FLAG_GRAPH.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU from different contexts race (calls of one
+ * context are ordered). */
+int eemd_run_batch_dev_noalloc(eemd_ctx* ctx, const double* x, const int64_t* offsets,
+                               int32_t nsig, int32_t E, const double* sigmas, const double* noise,
+                               const eemd_params* prm, double* imf_mean, double* residual,
+                               int32_t* n_modes, int32_t* nmode, int32_t* nsift,
+                               int32_t* per_mode_sifts, void* member_imfs, uint64_t* trace,
+                               void* stream);
+
 /* Times candidate launch configurations on the given device-resident batch
  * (x and noise as for eemd_run_batch_dev; the timed runs write to context
  * buffers) and returns the fastest: *best = *prm with threads,
