@@ -20,6 +20,7 @@ FLAG_TRACE = 2
 FLAG_PHASES = 4
 FLAG_GRAPH = 8
 FLAG_LEVEL_SUMS = 16
+FLAG_TUNE_BLOCK = 32
 
 class Params(C.Structure):
     _fields_ = [("fp64", C.c_int32), ("end_mode", C.c_int32), ("max_imf", C.c_int32),
@@ -188,7 +189,7 @@ class Engine:
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         sigmas = np.ascontiguousarray(sigmas, dtype=np.float64)
         nsig = len(offsets) - 1
-        prm.flags = prm.flags & (FLAG_PHASES | FLAG_GRAPH)
+        prm.flags = prm.flags & (FLAG_PHASES | FLAG_GRAPH | FLAG_LEVEL_SUMS | FLAG_TUNE_BLOCK)
         vp = C.c_void_p
         fn = self.lib.eemd_run_batch_dev_noalloc if noalloc else self.lib.eemd_run_batch_dev
         with _lock:

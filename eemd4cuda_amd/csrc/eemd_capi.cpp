@@ -120,7 +120,7 @@ struct GraphCache {
 // Launch configuration chosen by eemd_autotune for one batch shape.
 struct TuneEntry {
   std::vector<int64_t> key;
-  int threads, mpt, wave_windows;
+  int threads, mpt, wave_windows, block;
 };
 
 struct eemd_ctx {
@@ -143,6 +143,7 @@ struct eemd_ctx {
   // outputs of the autotuner's timed runs
   DevBuf tx, tavg, tres, tnm, tnmode, tnsift, tpm;
   DevBuf mlpw;  // BF16 boundary model (EEMD_END_LEARNED)
+  DevBuf mlpi;  // the model as one row per neuron (read by one-wave workgroups)
   bool have_model = false;
   GraphCache graph;
   std::vector<TuneEntry> tuned;
@@ -199,6 +200,7 @@ struct Plan {
   int threads = 0;
   int mpt = 0;
   int wave_windows = 0;
+  int block = 0;  // block-Thomas block size the launch uses
   bool tuned = false;
   int grid = 0;
   int64_t npad = 0, cap = 0, rows = 0, cols = 0;
@@ -206,10 +208,12 @@ struct Plan {
 };
 
 int auto_threads(int64_t nmax, bool fp64) {
-  // 1024-thread workgroups cap a lane at 128 VGPRs, below what the fused
-  // sift pass needs; 512 keeps every FP32 value in registers.  The FP64 pass
-  // holds twice the registers per sample and runs fastest at 256 (measured
-  // on the 2^20-sample, 256-member workload: 425 ms at 256, 525 ms at 512).
+  // Few members (the 64-thread rule in make_plan does not apply): a
+  // workgroup per member.  1024-thread workgroups cap a lane at 128 VGPRs,
+  // below what the fused sift pass needs; 512 keeps every FP32 value in
+  // registers.  The FP64 pass holds twice the registers per sample and runs
+  // fastest at 256 (measured on the 2^20-sample, 256-member workload: 425 ms
+  // at 256, 525 ms at 512).
   if (nmax <= 8192 || fp64) return 256;
   return 512;
 }
@@ -228,8 +232,9 @@ int validate(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, const
   if (p->end_mode == EEMD_END_LEARNED && !c->have_model)
     return fail(c, EEMD_ERR_BAD_ARG, "learned boundary needs a model (eemd_set_boundary_model)");
   if (p->halo < 0) return fail(c, EEMD_ERR_BAD_ARG, "halo must be >= 0");
-  if (p->threads != 0 && p->threads != 256 && p->threads != 512 && p->threads != 1024)
-    return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 256, 512 or 1024");
+  if (p->threads != 0 && p->threads != 64 && p->threads != 256 && p->threads != 512 &&
+      p->threads != 1024)
+    return fail(c, EEMD_ERR_BAD_ARG, "threads must be 0, 64, 256, 512 or 1024");
   if (p->members_per_task < 0) return fail(c, EEMD_ERR_BAD_ARG, "members_per_task must be >= 0");
   if (p->end_mode == EEMD_END_LEARNED && p->fp64 && p->threads == 1024)
     return fail(c, EEMD_ERR_BAD_ARG,
@@ -265,6 +270,7 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   for (int s = 0; s < nsig; ++s) P.nmax = std::max<int64_t>(P.nmax, offsets[s + 1] - offsets[s]);
   int want_threads = p->threads, want_mpt = p->members_per_task, want_ww = p->wave_windows;
   P.tuned = false;
+  P.block = p->block;
   if (want_threads == 0 && want_mpt == 0 && want_ww == 0) {
     const std::vector<int64_t> key = tune_key(offsets, nsig, E, p);
     for (const TuneEntry& t : c->tuned)
@@ -272,11 +278,22 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
         want_threads = t.threads;
         want_mpt = t.mpt;
         want_ww = t.wave_windows;
+        // the tuned block size only where the caller allows it
+        if (p->flags & EEMD_FLAG_TUNE_BLOCK) P.block = t.block;
         P.tuned = true;
         break;
       }
   }
-  P.threads = want_threads ? want_threads : auto_threads(P.nmax, p->fp64 != 0);
+  P.threads = want_threads;
+  if (!P.threads) {
+    // many members: one wave per member (64-thread workgroups, many per CU)
+    // when the batch fills every wave slot of the chip at least twice over
+    int b64 = 1;
+    hipError_t oe = sift_occupancy(p->fp64, 64, p->end_mode, &b64);
+    if (oe != hipSuccess) return hip_fail(c, oe, "occupancy query");
+    if ((int64_t)nsig * E >= 2 * (int64_t)std::max(b64, 1) * c->ncu) P.threads = 64;
+  }
+  if (!P.threads) P.threads = auto_threads(P.nmax, p->fp64 != 0);
   const int nw = P.threads / WAVE;
   P.wave_windows = want_ww == 0 ? 8 * nw : (want_ww < 0 ? -1 : want_ww);
 
@@ -351,11 +368,11 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   // scratch geometry
   P.npad = ((P.nmax + CHUNK - 1) / CHUNK) * CHUNK + CHUNK;
   P.cap = seg_cap(seg_len(P.nmax, nw));
-  if (p->block <= 0) {
+  if (P.block <= 0) {
     P.rows = P.npad / 2 + 2 * MAX_VIRT + 4;
     P.cols = 2;
   } else {
-    P.rows = (int64_t)p->block + 2 * (int64_t)p->halo;
+    P.rows = (int64_t)P.block + 2 * (int64_t)p->halo;
     // FP32: room for the wave-parallel window solve (3 x 64 x 64 floats per wave)
     if (!p->fp64) P.rows = std::max<int64_t>(P.rows, 3 * 64 * WAVE * 4 / (2 * 4 * WAVE));
     P.cols = P.threads;
@@ -458,7 +475,7 @@ void key_put_vec(std::vector<uint8_t>& k, const std::vector<V>& v) {
 // which enqueue_work fills in).
 SiftArgs sift_args(const Plan& P, const DevPtrs& D, const double* x, const double* noise,
                    const eemd_params* p, int32_t* nmode, int32_t* nsift, int32_t* per_mode,
-                   void* mimf, uint64_t* trace, const void* mlp) {
+                   void* mimf, uint64_t* trace, const void* mlp, const void* mlp_img) {
   SiftArgs A;
   std::memset(&A, 0, sizeof(A));  // padding included: A is part of the graph key
   A.x = x;
@@ -472,7 +489,7 @@ SiftArgs sift_args(const Plan& P, const DevPtrs& D, const double* x, const doubl
   A.min_extrema = p->min_extrema;
   A.end_mode = p->end_mode;
   A.tau = p->tau;
-  A.B = p->block;
+  A.B = P.block;
   A.S = p->halo;
   A.acc = D.acc;
   A.acc_cnt = D.acc_cnt;
@@ -488,6 +505,7 @@ SiftArgs sift_args(const Plan& P, const DevPtrs& D, const double* x, const doubl
   A.rows = P.rows;
   A.cols = P.cols;
   A.mlp = (p->end_mode == EEMD_END_LEARNED) ? mlp : nullptr;
+  A.mlp_img = (p->end_mode == EEMD_END_LEARNED) ? mlp_img : nullptr;
   A.nslots = P.grid;
   A.phase = (p->flags & EEMD_FLAG_PHASES) ? D.phase : nullptr;
   A.wave_windows = P.wave_windows;
@@ -513,7 +531,8 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
   D.acc_cnt = (int32_t*)c->acc_cnt.p;
   D.queue = (int32_t*)c->queue.p;
   D.phase = (uint64_t*)c->phase.p;
-  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, per_mode, mimf, trace, c->mlpw.p);
+  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, per_mode, mimf, trace, c->mlpw.p,
+                         c->mlpi.p);
   c->last_phase_grid = (p->flags & EEMD_FLAG_PHASES) ? P.grid : 0;
   // order this call after the previous one, whatever stream it used
   if (c->done_recorded) HIPCHK(c, hipStreamWaitEvent(st, c->done, 0), "stream wait");
@@ -532,6 +551,7 @@ int run_plan(eemd_ctx* c, const Plan& P, const double* x, const double* noise,
     key_put(key, P.grid);
     key_put(key, P.stride);
     key_put(key, P.wave_windows);
+    key_put(key, P.block);
     key_put(key, A);
     // the mean kernel's outputs are launch arguments outside SiftArgs
     key_put(key, avg);
@@ -661,7 +681,8 @@ int run_plan_noalloc(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t 
   D.acc = (double*)take(8 * std::max<int64_t>(P.acc_doubles, 1));
   D.scratch = take(P.stride * P.grid);
   if (o > cap) return fail(c, EEMD_ERR_BAD_ARG, "allocation-free scratch overflow");
-  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, pm, mimf, trace, c->mlpw.p);
+  SiftArgs A = sift_args(P, D, x, noise, p, nmode, nsift, pm, mimf, trace, c->mlpw.p,
+                         c->mlpi.p);
   c->last_phase_grid = 0;  // eemd_last_phases reads the context's phase buffer
   if (c->done_recorded) HIPCHK(c, hipStreamWaitEvent(st, c->done, 0), "stream wait");
   HIPCHK(c, upload_desc(P, D, st), "descriptor upload");
@@ -744,7 +765,7 @@ int eemd_ctx_destroy(eemd_ctx* c) {
   for (DevBuf* b : {&c->scratch, &c->acc, &c->acc_cnt, &c->queue, &c->phase, &c->hx, &c->hnoise,
                     &c->havg, &c->hres, &c->hnm, &c->hnmode, &c->hnsift, &c->hpm, &c->hmi, &c->htr,
                     &c->tx, &c->tavg, &c->tres, &c->tnm, &c->tnmode, &c->tnsift, &c->tpm,
-                    &c->mlpw})
+                    &c->mlpw, &c->mlpi})
     b->release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -880,19 +901,17 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
   HIPCHK(c, c->tnsift.ensure(sizeof(int32_t) * units), "tune alloc");
   HIPCHK(c, c->tpm.ensure(sizeof(int32_t) * units * prm->max_imf), "tune alloc");
 
-  // kernel time of one configuration (explicit threads / mpt / wave_windows)
-  auto timed = [&](int threads, int mpt, int ww, double& ms, int& slots) -> int {
+  // kernel time of one configuration (explicit threads / mpt / wave_windows / block)
+  auto timed = [&](int threads, int mpt, int ww, int block, double& ms) -> int {
     eemd_params q = *prm;
     q.threads = threads;
     q.members_per_task = mpt;
     q.wave_windows = ww;
+    q.block = block;
     q.flags = 0;
     Plan P;
     int r = make_plan(c, offsets, nsig, E, sigmas, noise, &q, false, P);
     if (r) return r;
-    int bpc = 1;
-    HIPCHK(c, sift_occupancy(q.fp64, P.threads, q.end_mode, &bpc), "occupancy query");
-    slots = std::max(1, bpc) * c->ncu;
     ms = 1e300;
     for (int k = 0; k < reps; ++k) {
       r = run_plan(c, P, x, noise, &q, (double*)c->tavg.p, (double*)c->tres.p, (int32_t*)c->tnm.p,
@@ -915,19 +934,30 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
   };
 
   // 1. workgroup size (automatic members per task, default wave threshold)
-  int bt = 0, bm = 0, bw = 0, slots = 1;
+  int bt = 0, bm = 0, bw = 0, bb = prm->block;
   double bms = 1e300;
-  for (int t : {256, 512, 1024}) {
+  for (int t : {64, 256, 512, 1024}) {
     // candidates validate() rejects are skipped (FP64 learned ends: the
     // model and the 1024-thread workgroup's LDS do not fit together)
     if (prm->fp64 && prm->end_mode == EEMD_END_LEARNED && t == 1024) continue;
     const int w = prm->fp64 ? -1 : 8 * (t / WAVE);
     double ms;
-    rc = timed(t, 0, w, ms, slots);
+    rc = timed(t, 0, w, bb, ms);
     if (rc) return rc;
     if (ms < bms) { bms = ms; bt = t; bm = 0; bw = w; }
   }
-  // 2. members per task: uniform choices, skipping those that leave slots
+  // 2. block size, where the caller allows it (results move by the seam
+  // truncation only); exact Thomas (block <= 0) stays exact
+  if ((prm->flags & EEMD_FLAG_TUNE_BLOCK) && prm->block > 0) {
+    for (int B : {128, 256, 512, 1024, 2048}) {
+      if (B == prm->block) continue;
+      double ms;
+      rc = timed(bt, 0, bw, B, ms);
+      if (rc) return rc;
+      if (ms < bms) { bms = ms; bb = B; }
+    }
+  }
+  // 3. members per task: uniform choices, skipping those that leave slots
   // idle or need far more accumulator memory than the automatic rule
   {
     int bpc = 1;
@@ -943,18 +973,18 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
       const int64_t ab = acc_bytes(bt, m);
       if (ab > 2 * auto_acc && ab > (int64_t(4) << 30)) continue;
       double ms;
-      rc = timed(bt, m, bw, ms, slots);
+      rc = timed(bt, m, bw, bb, ms);
       if (rc) return rc;
       if (ms < bms) { bms = ms; bm = m; }
     }
   }
-  // 3. FP32 wave-solve threshold
+  // 4. FP32 wave-solve threshold
   if (!prm->fp64) {
     const int nw = bt / WAVE;
     for (int w : {-1, 2 * nw, 32 * nw}) {
       if (w == bw) continue;
       double ms;
-      rc = timed(bt, bm, w, ms, slots);
+      rc = timed(bt, bm, w, bb, ms);
       if (rc) return rc;
       if (ms < bms) { bms = ms; bw = w; }
     }
@@ -966,13 +996,15 @@ int eemd_autotune(eemd_ctx* c, const double* x, const int64_t* offsets, int32_t 
       t.threads = bt;
       t.mpt = bm;
       t.wave_windows = bw;
+      t.block = bb;
       found = true;
     }
-  if (!found) c->tuned.push_back(TuneEntry{key, bt, bm, bw});
+  if (!found) c->tuned.push_back(TuneEntry{key, bt, bm, bw, bb});
   *best = *prm;
   best->threads = bt;
   best->members_per_task = bm;
   best->wave_windows = bw;
+  best->block = bb;
   if (best_ms) *best_ms = bms;
   return EEMD_OK;
 }
@@ -1017,10 +1049,27 @@ int eemd_set_boundary_model(eemd_ctx* c, const float* params) {
     if (!std::isfinite(params[i])) return fail(c, EEMD_ERR_BAD_ARG, "non-finite model parameter");
   std::vector<uint16_t> w(EEMD_MLP_NPARAM);
   for (int i = 0; i < EEMD_MLP_NPARAM; ++i) w[i] = f32_to_bf16(params[i]);
+  // one row per neuron, W^T[out][in + pad] per layer, then the biases
+  std::vector<uint16_t> img(MLP_LDS_N, 0);
+  {
+    const int gw[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4};
+    const int gb[4] = {MLP_B1, MLP_B2, MLP_B3, MLP_B4};
+    const int lw[4] = {MLP_LDS_W1, MLP_LDS_W2, MLP_LDS_W3, MLP_LDS_W4};
+    const int lb[4] = {MLP_LDS_B1, MLP_LDS_B2, MLP_LDS_B3, MLP_LDS_B4};
+    for (int l = 0; l < 4; ++l) {
+      const int nin = l == 0 ? MLP_IN : MLP_HID, nout = l == 3 ? MLP_OUT : MLP_HID;
+      for (int i = 0; i < nin; ++i)
+        for (int j = 0; j < nout; ++j) img[lw[l] + j * (nin + MLP_LDS_PAD) + i] = w[gw[l] + i * nout + j];
+      for (int j = 0; j < nout; ++j) img[lb[l] + j] = w[gb[l] + j];
+    }
+  }
   HIPCHK(c, hipSetDevice(c->device), "set device");
   HIPCHK(c, c->mlpw.ensure(sizeof(uint16_t) * EEMD_MLP_NPARAM), "model alloc");
+  HIPCHK(c, c->mlpi.ensure(sizeof(uint16_t) * MLP_LDS_N), "model alloc");
   HIPCHK(c, hipStreamSynchronize(c->stream), "model upload");
   HIPCHK(c, hipMemcpy(c->mlpw.p, w.data(), sizeof(uint16_t) * EEMD_MLP_NPARAM,
+                      hipMemcpyHostToDevice), "model upload");
+  HIPCHK(c, hipMemcpy(c->mlpi.p, img.data(), sizeof(uint16_t) * MLP_LDS_N,
                       hipMemcpyHostToDevice), "model upload");
   c->have_model = true;
   return EEMD_OK;

@@ -131,7 +131,8 @@ struct SiftArgs {
   int64_t npad;             // padded max length
   int64_t cap;              // knot slots per segment region
   int64_t rows, cols;       // window scratch shape
-  const void* mlp;          // learned-boundary model (optional)
+  const void* mlp;          // learned-boundary model (optional), BF16 W[in][out] per layer
+  const void* mlp_img;      // the same model as one row per neuron (MLP_LDS_* layout)
   uint64_t* phase;          // optional per-workgroup phase clocks [nslots][4]
   int32_t wave_windows;     // FP32: wave-parallel window solve when windows <= this
   int32_t nslots;           // number of workgroups (scratch slots)

@@ -1953,8 +1953,14 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
 // ------------------------------------------------------------------------
 __device__ __forceinline__ uint64_t clk() { return wall_clock64(); }
 
+// Waves per SIMD the kernel is register-limited to.  A 64-thread workgroup
+// (one wave, one member at a time) in FP32 keeps two waves per SIMD, as the
+// 512-thread workgroup must; larger workgroups get what their size implies.
 template <typename T, int NT>
-__global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
+constexpr int sift_min_waves() { return (NT == 64 && sizeof(T) == 4) ? 2 : 1; }
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, (sift_min_waves<T, NT>())) void sift_kernel(SiftArgs A) {
   constexpr int NW = NT / WAVE;
   __shared__ Shared<T, NW> S;
   __shared__ LearnedLDS LL;
@@ -1990,22 +1996,30 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
     tc = t_;                              \
   }
 
+  // The BF16 model, one row per neuron (the layout of MLP_LDS_*): a copy in
+  // LDS for the workgroup's lifetime, except for one-wave workgroups, which
+  // read the same image from global memory (L2-resident) so that many of them
+  // fit on a CU.
+  const uint16_t* model = model_lds;
   if (A.end_mode == END_LEARNED) {
-    // the BF16 model stays in LDS for the workgroup's lifetime, one row per
-    // neuron (global layout W[in][out] read coalesced)
-    const uint16_t* g = reinterpret_cast<const uint16_t*>(A.mlp);
-    const int gw[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4};
-    const int gb[4] = {MLP_B1, MLP_B2, MLP_B3, MLP_B4};
-    const int lw[4] = {MLP_LDS_W1, MLP_LDS_W2, MLP_LDS_W3, MLP_LDS_W4};
-    const int lb[4] = {MLP_LDS_B1, MLP_LDS_B2, MLP_LDS_B3, MLP_LDS_B4};
+    if constexpr (NT == WAVE) {
+      model = reinterpret_cast<const uint16_t*>(A.mlp_img);
+    } else {
+      // global layout W[in][out] read coalesced
+      const uint16_t* g = reinterpret_cast<const uint16_t*>(A.mlp);
+      const int gw[4] = {MLP_W1, MLP_W2, MLP_W3, MLP_W4};
+      const int gb[4] = {MLP_B1, MLP_B2, MLP_B3, MLP_B4};
+      const int lw[4] = {MLP_LDS_W1, MLP_LDS_W2, MLP_LDS_W3, MLP_LDS_W4};
+      const int lb[4] = {MLP_LDS_B1, MLP_LDS_B2, MLP_LDS_B3, MLP_LDS_B4};
 #pragma unroll
-    for (int l = 0; l < 4; ++l) {
-      const int nin = l == 0 ? MLP_IN : MLP_HID, nout = l == 3 ? MLP_OUT : MLP_HID;
-      for (int k = tid; k < nin * nout; k += NT) {
-        const int i = k / nout, j = k - i * nout;
-        model_lds[lw[l] + j * (nin + MLP_LDS_PAD) + i] = g[gw[l] + k];
+      for (int l = 0; l < 4; ++l) {
+        const int nin = l == 0 ? MLP_IN : MLP_HID, nout = l == 3 ? MLP_OUT : MLP_HID;
+        for (int k = tid; k < nin * nout; k += NT) {
+          const int i = k / nout, j = k - i * nout;
+          model_lds[lw[l] + j * (nin + MLP_LDS_PAD) + i] = g[gw[l] + k];
+        }
+        for (int j = tid; j < nout; j += NT) model_lds[lb[l] + j] = g[gb[l] + j];
       }
-      for (int j = tid; j < nout; j += NT) model_lds[lb[l] + j] = g[gb[l] + j];
     }
   }
 
@@ -2075,7 +2089,7 @@ __global__ __launch_bounds__(NT) void sift_kernel(SiftArgs A) {
           }
           __syncthreads();
           if (A.end_mode == END_LEARNED) {
-            learned_ends<T, NW, NT>(S, LL, M, cur, model_lds);
+            learned_ends<T, NW, NT>(S, LL, M, cur, model);
             __syncthreads();
           }
           PHASE(3);
@@ -2232,8 +2246,8 @@ __global__ void noise_kernel(uint64_t seed, uint32_t sig, uint32_t m0, int E, in
 // host-side launchers (called from the C-ABI)
 // ------------------------------------------------------------------------
 // Dynamic LDS of a launch: the BF16 boundary model in learned mode.
-static size_t sift_dyn_lds(int end_mode) {
-  return end_mode == END_LEARNED ? MLP_LDS_BYTES : 0;
+static size_t sift_dyn_lds(int end_mode, int nt) {
+  return (end_mode == END_LEARNED && nt != WAVE) ? MLP_LDS_BYTES : 0;
 }
 
 template <typename T, int NT>
@@ -2246,7 +2260,7 @@ static hipError_t allow_model_lds() {
 
 template <typename T, int NT>
 static hipError_t launch_sift_t(const SiftArgs& A, int grid, hipStream_t st) {
-  const size_t lds = sift_dyn_lds(A.end_mode);
+  const size_t lds = sift_dyn_lds(A.end_mode, NT);
   if (lds) {
     const hipError_t e = allow_model_lds<T, NT>();
     if (e != hipSuccess) return e;
@@ -2258,12 +2272,14 @@ static hipError_t launch_sift_t(const SiftArgs& A, int grid, hipStream_t st) {
 hipError_t launch_sift(const SiftArgs& A, int fp64, int nt, int grid, hipStream_t st) {
   if (fp64) {
     switch (nt) {
+      case 64: return launch_sift_t<double, 64>(A, grid, st);
       case 256: return launch_sift_t<double, 256>(A, grid, st);
       case 512: return launch_sift_t<double, 512>(A, grid, st);
       case 1024: return launch_sift_t<double, 1024>(A, grid, st);
     }
   } else {
     switch (nt) {
+      case 64: return launch_sift_t<float, 64>(A, grid, st);
       case 256: return launch_sift_t<float, 256>(A, grid, st);
       case 512: return launch_sift_t<float, 512>(A, grid, st);
       case 1024: return launch_sift_t<float, 1024>(A, grid, st);
@@ -2275,21 +2291,25 @@ hipError_t launch_sift(const SiftArgs& A, int fp64, int nt, int grid, hipStream_
 hipError_t sift_occupancy(int fp64, int nt, int end_mode, int* blocks_per_cu) {
   const void* f = nullptr;
   if (fp64) {
+    if (nt == 64) f = (const void*)sift_kernel<double, 64>;
     if (nt == 256) f = (const void*)sift_kernel<double, 256>;
     if (nt == 512) f = (const void*)sift_kernel<double, 512>;
     if (nt == 1024) f = (const void*)sift_kernel<double, 1024>;
   } else {
+    if (nt == 64) f = (const void*)sift_kernel<float, 64>;
     if (nt == 256) f = (const void*)sift_kernel<float, 256>;
     if (nt == 512) f = (const void*)sift_kernel<float, 512>;
     if (nt == 1024) f = (const void*)sift_kernel<float, 1024>;
   }
   if (!f) return hipErrorInvalidValue;
-  const size_t lds = sift_dyn_lds(end_mode);
+  const size_t lds = sift_dyn_lds(end_mode, nt);
   if (lds) {
-    const hipError_t e = fp64 ? (nt == 256 ? allow_model_lds<double, 256>()
+    const hipError_t e = fp64 ? (nt == 64    ? allow_model_lds<double, 64>()
+                                 : nt == 256 ? allow_model_lds<double, 256>()
                                  : nt == 512 ? allow_model_lds<double, 512>()
                                              : allow_model_lds<double, 1024>())
-                              : (nt == 256 ? allow_model_lds<float, 256>()
+                              : (nt == 64    ? allow_model_lds<float, 64>()
+                                 : nt == 256 ? allow_model_lds<float, 256>()
                                  : nt == 512 ? allow_model_lds<float, 512>()
                                              : allow_model_lds<float, 1024>());
     if (e != hipSuccess) return e;

@@ -60,6 +60,10 @@ extern "C" {
                                    divided by the member counts) and residual is x minus
                                    their sum: for callers that combine batches of one
                                    ensemble, as emd_par.py:234-250 does */
+#define EEMD_FLAG_TUNE_BLOCK 32 /* eemd_autotune may change the block-Thomas block size, and
+                                   a later call with this flag and automatic launch
+                                   parameters uses the tuned block (results then move by
+                                   the seam truncation, at most 8 rho^S relative) */
 
 typedef struct eemd_ctx eemd_ctx;
 
@@ -71,7 +75,7 @@ typedef struct eemd_params {
   int32_t min_extrema;      /* stop when the residual has fewer extrema (emd_ref.py:123) */
   int32_t block;            /* decay-seamed block-Thomas block B; <= 0: exact Thomas */
   int32_t halo;             /* seam halo S in knots */
-  int32_t threads;          /* workgroup size 256 / 512 / 1024; 0: automatic */
+  int32_t threads;          /* workgroup size 64 / 256 / 512 / 1024; 0: automatic */
   int32_t members_per_task; /* ensemble members per queue entry; 0: automatic */
   int32_t flags;            /* EEMD_FLAG_* */
   double tau;               /* SD stopping tolerance (emd_ref.py:140-146); <= 0: fixed sifts */
@@ -155,13 +159,16 @@ int eemd_run_batch_dev_noalloc(eemd_ctx* ctx, const double* x, const int64_t* of
 /* Times candidate launch configurations on the given device-resident batch
  * (x and noise as for eemd_run_batch_dev; the timed runs write to context
  * buffers) and returns the fastest: *best = *prm with threads,
- * members_per_task and (FP32) wave_windows set, *best_ms its kernel time.
- * The context remembers the choice for this batch shape (lengths, E, FP mode,
- * IMF/sift counts, block, halo, end mode) and applies it to later calls whose
- * threads, members_per_task and wave_windows are all 0.  Coordinate search:
- * workgroup size, then members per task, then the FP32 wave-solve threshold;
- * each candidate runs `reps` times and keeps its fastest run.  The reference
- * picks its launch shape by hand (emd_gpu_v5_dsbt.py:343, K / nthreads / B
+ * members_per_task, (FP32) wave_windows and, with EEMD_FLAG_TUNE_BLOCK in
+ * prm->flags, block set; *best_ms its kernel time.  The context remembers the
+ * choice for this batch shape (lengths, E, FP mode, IMF/sift counts, block,
+ * halo, end mode) and applies it to later calls whose threads,
+ * members_per_task and wave_windows are all 0 (the block only when such a
+ * call also sets EEMD_FLAG_TUNE_BLOCK).  Coordinate search: workgroup size
+ * (64 threads: one wave, one member at a time), then the block size, then
+ * members per task, then the FP32 wave-solve threshold; each candidate runs
+ * `reps` times and keeps its fastest run.  The reference picks its launch
+ * shape and block size by hand (emd_gpu_v5_dsbt.py:343, K / nthreads / B
  * arguments of gpu_eemd_full). */
 int eemd_autotune(eemd_ctx* ctx, const double* x, const int64_t* offsets, int32_t nsig, int32_t E,
                   const double* sigmas, const double* noise, const eemd_params* prm, int32_t reps,

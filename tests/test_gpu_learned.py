@@ -94,8 +94,8 @@ def test_mlp_training_reduces_loss(eng):
     assert last < 0.3 * first
 
 
-@pytest.mark.parametrize("B,S", [(0, 30), (256, 30)])
-def test_learned_ends_fp64_match_oracle(eng, B, S):
+@pytest.mark.parametrize("B,S,threads", [(0, 30, 0), (256, 30, 0), (256, 30, 64)])
+def test_learned_ends_fp64_match_oracle(eng, B, S, threads):
     from eemd4cuda_amd import _lib, learned
     rng = np.random.default_rng(4)
     N = 6000
@@ -104,7 +104,7 @@ def test_learned_ends_fp64_match_oracle(eng, B, S):
     z = rng.standard_normal((3, N))
     sigma = 0.2 * np.std(x)
     p = mlp_ref.random_params(rng, scale=0.3)
-    print("case: learned ends fp64 N=6000 E=3 B=%d" % B, flush=True)
+    print("case: learned ends fp64 N=6000 E=3 B=%d threads=%d" % (B, threads), flush=True)
     learned.install(p, eng)
     prm = _lib.default_params()
     prm.fp64 = 1
@@ -113,6 +113,7 @@ def test_learned_ends_fp64_match_oracle(eng, B, S):
     prm.block = B
     prm.halo = S
     prm.members_per_task = 1
+    prm.threads = threads
     out = eng.run_batch(x, np.array([0, N]), 3, np.array([sigma]), prm, noise=z,
                         member_imfs=True, trace=True)
     orc.set_model(p)

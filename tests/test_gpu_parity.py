@@ -73,7 +73,8 @@ def test_fp64_three_tone_matches_oracle_and_reference(eng, golden):
     assert np.max(np.abs(out["resid"] - g["resid"])) <= 1e-10
 
 
-@pytest.mark.parametrize("B,S,threads", [(64, 30, 256), (100, 14, 1024), (1024, 30, 512)])
+@pytest.mark.parametrize("B,S,threads", [(64, 30, 256), (100, 14, 1024), (1024, 30, 512),
+                                         (256, 30, 64)])
 def test_fp64_dsbt_matches_oracle(eng, B, S, threads):
     rng = np.random.default_rng(7)
     N = 20000
@@ -240,7 +241,7 @@ def test_workgroup_sizes_agree(eng):
     rng = np.random.default_rng(2)
     x = np.sin(np.arange(30000) * 0.003) + 0.2 * rng.standard_normal(30000)
     outs = []
-    for nt in (256, 512, 1024):
+    for nt in (256, 64, 512, 1024):
         print("case: threads", nt, flush=True)
         outs.append(run1(eng, x, None, 0.1, E=3, B=256, S=30, threads=nt))
     for o in outs[1:]:
