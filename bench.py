@@ -474,6 +474,14 @@ def main():
         tr = json.loads(prof.read_text())
         out["roofline"]["traffic"] = tr.get("f64_bytes_per_launch")
         out["fp32"]["roofline"]["traffic"] = tr.get("f32_bytes_per_launch")
+    if not a.no_legs and a.workload == "uneven":
+        _log("leg: batched vs plain on the whole uneven batch")
+        out["batched_vs_plain"] = {"workload": wl["label"],
+                                   "f64": comparators(wl, "f64", 1, 30, tune_block=True),
+                                   "f32": comparators(wl, "f32", 1, a.halo32, tune_block=True)}
+    if not a.no_legs and a.workload == "chirp_scale":
+        _log("leg: learned vs mirror ends on the whole chirp/AM set")
+        out["learned_vs_mirror"] = learned_leg(wl, chirp_subset(32), a.steps, a.warmup, a.halo32)
     if not a.no_legs and a.workload == "audio_mix":
         _log("leg: batched vs plain on 128 uneven signals")
         sub = uneven_subset(128)

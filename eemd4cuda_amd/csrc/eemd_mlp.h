@@ -130,4 +130,29 @@ __device__ __forceinline__ float mlp_half_row(const uint16_t* row, float acc, co
   return acc;
 }
 
+// mlp_half_row for a compile-time n: all of the row's weights are loaded
+// before the fma chain, so the loads are in flight together (the row may be
+// in global memory); the arithmetic is mlp_half_row's.
+template <int N>
+__device__ __forceinline__ float mlp_half_row_n(const uint16_t* row, float acc, const float* in) {
+  static_assert(N % 8 == 0, "rows are read in 16-byte pieces");
+  uint4 wq[N / 8];
+#pragma unroll
+  for (int q = 0; q < N / 8; ++q) wq[q] = *reinterpret_cast<const uint4*>(row + 8 * q);
+#pragma unroll
+  for (int q = 0; q < N / 8; ++q) {
+    const float4 xa = *reinterpret_cast<const float4*>(in + 8 * q);
+    const float4 xb = *reinterpret_cast<const float4*>(in + 8 * q + 4);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].x << 16), xa.x, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].x & 0xFFFF0000u), xa.y, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].y << 16), xa.z, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].y & 0xFFFF0000u), xa.w, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].z << 16), xb.x, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].z & 0xFFFF0000u), xb.y, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].w << 16), xb.z, acc);
+    acc = fmaf(__builtin_bit_cast(float, wq[q].w & 0xFFFF0000u), xb.w, acc);
+  }
+  return acc;
+}
+
 }  // namespace eemd

@@ -1385,7 +1385,8 @@ __device__ __forceinline__ void learned_ends(Shared<T, NW>& S, LearnedLDS& L, co
         const float* in = l == 0 ? L.x[end] : L.h[(l - 1) & 1][end];
         const uint16_t* row = w + wrow + jn * (nin + MLP_LDS_PAD) + half * (nin / 2);
         const float acc0 = half ? 0.0f : bf16_to_f32(w[brow + jn]);
-        part = mlp_half_row(row, acc0, in + half * (nin / 2), nin / 2);
+        part = (l == 0) ? mlp_half_row_n<MLP_IN / 2>(row, acc0, in + half * (nin / 2))
+                        : mlp_half_row_n<MLP_HID / 2>(row, acc0, in + half * (nin / 2));
       }
       const float z = part + __shfl_xor(part, 32);
       if (l < 3) {
