@@ -255,3 +255,38 @@ def test_autotune_fp64_learned_ends(eng):
     best, ms = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
     assert best.threads in (256, 512) and ms > 0
     p.end_mode = _lib.END_MODES["mirror"]
+
+
+def test_autotune_block_only_with_flag(eng):
+    """The tuner changes the block size only under EEMD_FLAG_TUNE_BLOCK, and a
+    later automatic call uses the tuned block only when it sets the flag too:
+    its result then equals an explicit call with that block."""
+    import torch
+    from eemd4cuda_amd import _lib
+    x, offs, sig = _batch(11, [6000, 20000, 3000])
+    E, mi = 10, 5
+    xd = torch.from_numpy(x).to(torch.device("cuda", 0))
+    p = _params(0, B=1024, max_imf=mi)
+    print("case: autotune without the block flag", flush=True)
+    best, _ = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
+    assert best.block == 1024
+    p.flags = _lib.FLAG_TUNE_BLOCK
+    print("case: autotune with the block flag", flush=True)
+    best, _ = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
+    p.flags = 0
+    assert best.block in (128, 256, 512, 1024, 2048)
+    exp = _params(0, B=best.block, max_imf=mi)
+    exp.threads, exp.members_per_task, exp.wave_windows = (best.threads, best.members_per_task,
+                                                           best.wave_windows)
+    a = _dev_run(eng, xd, offs, E, sig, exp, mi)
+    auto = _params(0, B=1024, max_imf=mi)
+    auto.flags = _lib.FLAG_TUNE_BLOCK
+    b = _dev_run(eng, xd, offs, E, sig, auto, mi)
+    assert np.array_equal(a["avg"], b["avg"]) and np.array_equal(a["nsift"], b["nsift"])
+    # without the flag the automatic call keeps the requested block
+    c = _dev_run(eng, xd, offs, E, sig, _params(0, B=1024, max_imf=mi), mi)
+    ref = _params(0, B=1024, max_imf=mi)
+    ref.threads, ref.members_per_task, ref.wave_windows = (best.threads, best.members_per_task,
+                                                           best.wave_windows)
+    d = _dev_run(eng, xd, offs, E, sig, ref, mi)
+    assert np.array_equal(c["avg"], d["avg"])
