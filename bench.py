@@ -294,6 +294,38 @@ def comparators(wl, fp, steps, halo, tune_block):
     return out
 
 
+def reference_sweep_leg():
+    """The reference's own GPU benchmark (its published wall times are in
+    BASELINE.md): EEMD of the 441,000-sample 10-octave sweep (refsignal.py,
+    emd_par.py:307-318) through decompose(method='gpu') with the reference's
+    settings: numpy noise default_rng(0), eps 0.2, SD stop tau 0.25 (at most
+    50 sifts), 16 IMFs, natural ends, block-Thomas B=250 / S=30, FP64.  Wall
+    time of the whole call, host noise and PCIe copies included, as the
+    reference measures it."""
+    from eemd4cuda_amd import emd_par, signals
+    x = signals.reference_sweep()
+    out = {"signal": "reference 10-octave sweep, N=441000", "published_wall_s": {
+        "rtx4070_laptop": {"8": 0.83, "32": 1.77, "64": 2.54, "128": 11.08},
+        "gb10": {"8": 1.73, "32": 3.87, "64": 4.57, "128": 6.22}}}
+    kw = dict(method="gpu", eps=0.2, tau=0.25, max_sifts=50, max_imf=16, end="natural", B=250,
+              S=30, seed=0)
+    emd_par.decompose(x, E=8, **kw)   # warm-up (first use of the context's buffers)
+    for E in (8, 32, 64, 128):
+        t0 = time.perf_counter()
+        r = emd_par.decompose(x, E=E, **kw)
+        wall = time.perf_counter() - t0
+        out["E%d" % E] = dict(wall_s=wall, n_modes=int(r.n_modes),
+                              total_sifts=int(r.stats["total_sifts"]),
+                              sample_sifts_per_s=r.stats["sample_sifts"] / wall)
+        # the same with the noise drawn in-kernel (Philox) instead of on the host
+        t0 = time.perf_counter()
+        r = emd_par.decompose(x, E=E, noise="philox", **kw)
+        wall = time.perf_counter() - t0
+        out["E%d_philox" % E] = dict(wall_s=wall, total_sifts=int(r.stats["total_sifts"]),
+                                     sample_sifts_per_s=r.stats["sample_sifts"] / wall)
+    return out
+
+
 def cpu_baseline(wl, budget_cores: int):
     """The reference's CPU path (numpy restatement) on one member per core."""
     from concurrent.futures import ProcessPoolExecutor
@@ -491,6 +523,9 @@ def main():
         _log("leg: learned vs mirror ends on 512 chirp/AM signals")
         out["learned_vs_mirror"] = learned_leg(chirp_subset(512), chirp_subset(32), 1, 1,
                                                a.halo32)
+    if not a.no_legs and a.workload == "audio_mix":
+        _log("leg: the reference's own benchmark (441,000-sample sweep)")
+        out["reference_benchmark"] = reference_sweep_leg()
     if not a.no_cpu and len(wl["xs"]) >= 1:
         _log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_cores)
