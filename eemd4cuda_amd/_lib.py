@@ -6,6 +6,7 @@ raises.  One :class:`Engine` (device context) is created lazily per process.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import threading
 from pathlib import Path
 
@@ -38,20 +39,37 @@ _lib = None
 _lock = threading.Lock()
 
 
+def _runtime_choice() -> str:
+    """Which HIP runtime the library binds to: EEMD_HIP_RUNTIME=torch | system.
+
+    A process holds one HIP runtime.  PyTorch ships its own libamdhip64 with
+    the same soname as ROCm's; whichever is loaded first serves the process,
+    and a second copy loaded later starts a runtime that sees no GPU.
+    'torch' (the default when PyTorch is installed) loads PyTorch's runtime
+    first, so device memory from torch tensors and the engine's are one;
+    'system' binds ROCm's runtime and leaves torch out of the process."""
+    want = os.environ.get("EEMD_HIP_RUNTIME", "").strip().lower()
+    if want not in ("", "torch", "system"):
+        raise ValueError("EEMD_HIP_RUNTIME must be 'torch' or 'system'")
+    if want == "system":
+        return "system"
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        if want == "torch":
+            raise RuntimeError("EEMD_HIP_RUNTIME=torch but PyTorch is not importable")
+        return "system"
+    return "torch"
+
+
 def load(path: Path | None = None):
-    """Load the shared library (raises if it has not been built)."""
+    """Load the shared library (raises if it has not been built).  The HIP
+    runtime it binds to is chosen by _runtime_choice() before the load."""
     global _lib
     if _lib is not None:
         return _lib
     p = Path(path) if path else lib_path()
-    # One HIP runtime per process.  PyTorch ships its own libamdhip64 (same
-    # soname as ROCm's); loading the engine first would bind ROCm's copy and
-    # a later torch import would start a second runtime that sees no GPU.
-    # With torch imported first the engine binds to torch's runtime.
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
+    runtime = _runtime_choice()
     if not p.exists():
         raise RuntimeError(
             f"{p} not found: build it with `python -m eemd4cuda_amd.build` (hipcc, gfx950)")
@@ -80,6 +98,7 @@ def load(path: Path | None = None):
                  "eemd_noise", "eemd_last_timing", "eemd_last_phases", "eemd_autotune",
                  "eemd_graph_stats", "eemd_mem_info"):
         getattr(L, name).restype = C.c_int
+    L._eemd_runtime = runtime
     _lib = L
     return L
 

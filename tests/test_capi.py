@@ -122,3 +122,16 @@ def test_signals_shapes():
     assert a.shape == (4096,) and np.isfinite(a).all()
     b = signals.chirp_am_batch(2, 1024, seed=3)
     assert b.shape == (2, 1024)
+
+
+def test_hip_runtime_choice(monkeypatch):
+    """The HIP runtime the library binds to is explicit: EEMD_HIP_RUNTIME
+    selects it, anything else is rejected."""
+    from eemd4cuda_amd import _lib
+    monkeypatch.setenv("EEMD_HIP_RUNTIME", "system")
+    assert _lib._runtime_choice() == "system"
+    monkeypatch.setenv("EEMD_HIP_RUNTIME", "torch")
+    assert _lib._runtime_choice() == "torch"
+    monkeypatch.setenv("EEMD_HIP_RUNTIME", "bogus")
+    with pytest.raises(ValueError):
+        _lib._runtime_choice()
