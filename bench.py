@@ -452,12 +452,16 @@ def main():
         wl["max_imf"] = a.max_imf
     halo = {"f64": 30, "f32": a.halo32}
     if a.only:
-        res, _ = time_mode(wl, a.only, a.steps, a.warmup, halo=halo[a.only], threads=a.threads,
-                           mpt=a.mpt, wave_windows=a.wave_windows)
-        print(json.dumps(dict(fp=a.only, workload=wl["label"],
-                              **{k: res[k] for k in ("value", "ms_per_step", "sift_kernel_ms",
-                                                     "threads", "grid", "roofline")})),
-              flush=True)
+        res, run = time_mode(wl, a.only, a.steps, a.warmup, halo=halo[a.only], threads=a.threads,
+                             mpt=a.mpt, wave_windows=a.wave_windows)
+        line = dict(fp=a.only, workload=wl["label"],
+                    **{k: res[k] for k in ("value", "ms_per_step", "sift_kernel_ms",
+                                           "threads", "grid", "roofline")})
+        if a.phases:
+            run.step(phases=True)
+            line["phases_ms"] = dict(zip(("solve", "pass", "fixup", "other"),
+                                         run.eng.last_phases()))
+        print(json.dumps(line), flush=True)
         return
     _log("timing f64 (headline)")
     f64, r64 = time_mode(wl, "f64", a.steps, a.warmup, halo=30, threads=a.threads, mpt=a.mpt)

@@ -1798,40 +1798,80 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
       }
     }
     const int nl_ev = S.nl[ev], p_ev = S.p[ev];
+    // one row of the forward sweep (emd_gpu_v5_dsbt.py:58-66, the system of
+    // emd_ref.py:31-48 built from the knots)
+    auto row = [&](int q, T2 kn) {
+      const double ct = (double)kn.x, cy = (double)kn.y;
+      const double H0 = bt - at;
+      const double H1 = ct - bt;
+      const double d = 2.0 * (H0 + H1);
+      const double l = (q >= 1) ? H0 : 0.0;
+      // no upper coupling on the window's last row (the reference's 0.0) or
+      // on the system's last row: a zero numerator gives that zero (den > 0),
+      // so the division needs no branch
+      const double uu = (q <= n - 2 && q != w1) ? H1 : 0.0;
+      double sl1, c, xv;
+      const double den = d - l * cprev;
+      if constexpr (std::is_same<T, double>::value) {
+        sl1 = (cy - by) / H1;
+        const double r = 6.0 * (sl1 - sl0);
+        c = uu / den;
+        xv = (r - l * xprev) / den;
+      } else {
+        sl1 = qdiv<T>(cy - by, H1);
+        const double r = 6.0 * (sl1 - sl0);
+        const double inv = __builtin_amdgcn_rcp(den);
+        c = uu * inv;
+        xv = (r - l * xprev) * inv;
+      }
+      T2 cx;
+      cx.x = (T)c;
+      cx.y = (T)xv;
+      wcx[(int64_t)(q - w0) * M.cols + col] = cx;
+      cprev = c;
+      xprev = xv;
+      sl0 = sl1;
+      at = bt;
+      ay = by;
+      bt = ct;
+      by = cy;
+    };
 #pragma unroll 1
     for (int q0 = w0; q0 <= w1; q0 += PF) {
       T2 fk[PF];
-      bool wide = false;
-      if constexpr (std::is_same<T, float>::value) {
-        // the next PF knots are consecutive slots of one real run inside the
-        // window: read them 2 pairs per 16-byte load (half the requests of
-        // per-pair loads; each lane streams its own window, so these are
-        // uncoalesced and request-bound)
-        const int g = ks.e - nl_ev;
-        wide = g >= 0 && g + PF < p_ev && ks.rem > PF && enext + PF - 1 <= elast;
-      }
+      // the next PF knots are consecutive slots of one real run inside the
+      // window: read them directly (FP32: 2 pairs per 16-byte load, half the
+      // requests of per-pair loads; each lane streams its own window, so
+      // these are uncoalesced and request-bound)
+      const int g = ks.e - nl_ev;
+      const bool wide = g >= 0 && g + PF < p_ev && ks.rem > PF && enext + PF - 1 <= elast;
       if (wide) {
         const T2* src = tyb + ks.slot;
-        if ((ks.slot & 1) == 0) {
+        if constexpr (std::is_same<T, float>::value) {
+          if ((ks.slot & 1) == 0) {
 #pragma unroll
-          for (int u = 0; u < PF; u += 2) {
-            const float4 v = *reinterpret_cast<const float4*>(src + u);
-            fk[u].x = v.x;
-            fk[u].y = v.y;
-            fk[u + 1].x = v.z;
-            fk[u + 1].y = v.w;
+            for (int u = 0; u < PF; u += 2) {
+              const float4 v = *reinterpret_cast<const float4*>(src + u);
+              fk[u].x = v.x;
+              fk[u].y = v.y;
+              fk[u + 1].x = v.z;
+              fk[u + 1].y = v.w;
+            }
+          } else {
+            fk[0] = src[0];
+#pragma unroll
+            for (int u = 1; u < PF - 1; u += 2) {
+              const float4 v = *reinterpret_cast<const float4*>(src + u);
+              fk[u].x = v.x;
+              fk[u].y = v.y;
+              fk[u + 1].x = v.z;
+              fk[u + 1].y = v.w;
+            }
+            fk[PF - 1] = src[PF - 1];
           }
         } else {
-          fk[0] = src[0];
 #pragma unroll
-          for (int u = 1; u < PF - 1; u += 2) {
-            const float4 v = *reinterpret_cast<const float4*>(src + u);
-            fk[u].x = v.x;
-            fk[u].y = v.y;
-            fk[u + 1].x = v.z;
-            fk[u + 1].y = v.w;
-          }
-          fk[PF - 1] = src[PF - 1];
+          for (int u = 0; u < PF; ++u) fk[u] = src[u];
         }
         ks.slot += PF;
         ks.rem -= PF;
@@ -1849,40 +1889,15 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
           }
         }
       }
+      if (q0 + PF - 1 <= w1) {
+        // a whole group of rows: straight-line code, so the work of a row
+        // (and the slopes of later rows) overlaps
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        const int q = q0 + u;
-        if (q <= w1) {
-          const double ct = (double)nk[u].x, cy = (double)nk[u].y;
-          const double H0 = bt - at;
-          const double H1 = ct - bt;
-          const double d = 2.0 * (H0 + H1);
-          const double l = (q >= 1) ? H0 : 0.0;
-          const double uu = (q <= n - 2) ? H1 : 0.0;
-          const double sl1 = qdiv<T>(cy - by, H1);
-          const double r = 6.0 * (sl1 - sl0);
-          const double den = d - l * cprev;
-          double c, xv;
-          if constexpr (std::is_same<T, double>::value) {
-            c = (q == w1) ? 0.0 : uu / den;
-            xv = (r - l * xprev) / den;
-          } else {
-            const double inv = __builtin_amdgcn_rcp(den);
-            c = (q == w1) ? 0.0 : uu * inv;
-            xv = (r - l * xprev) * inv;
-          }
-          T2 cx;
-          cx.x = (T)c;
-          cx.y = (T)xv;
-          wcx[(int64_t)(q - w0) * M.cols + col] = cx;
-          cprev = c;
-          xprev = xv;
-          sl0 = sl1;
-          at = bt;
-          ay = by;
-          bt = ct;
-          by = cy;
-        }
+        for (int u = 0; u < PF; ++u) row(q0 + u, nk[u]);
+      } else {
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+          if (q0 + u <= w1) row(q0 + u, nk[u]);
       }
 #pragma unroll
       for (int u = 0; u < PF; ++u) nk[u] = fk[u];
@@ -1932,14 +1947,32 @@ __device__ __forceinline__ void solve(Shared<T, NW>& S, const Mem<T>& M, int cur
         const int q = q0 - PF - u;
         fx[u] = wcx[(int64_t)((q < w0 ? w0 : q) - w0) * M.cols + col];
       }
+      // a whole group of owned rows whose knots are consecutive slots of one
+      // real run (the window's interior): straight-line rows, direct slots
+      const int g = kr.e - nl;
+      if (q0 - PF + 1 >= w0 && q0 - PF + 1 >= o0 && q0 <= o1 && g < p && kr.left >= PF) {
+        if (pend_s >= 0) mb[pend_s] = pend_v;
+        pend_s = -1;
+        T* const ms = mb + kr.slot;
 #pragma unroll
-      for (int u = 0; u < PF; ++u) {
-        const int q = q0 - u;
-        if (q >= w0) {
+        for (int u = 0; u < PF; ++u) {
           const double xi = (double)cx[u].y - (double)cx[u].x * xn;
-          const int64_t sq = kr_prev(kr, S, M, cur, ev, nl, p);
-          if (q >= o0 && q <= o1) put_m(sq, (T)xi);
+          ms[-u] = (T)xi;
           xn = xi;
+        }
+        kr.slot -= PF;
+        kr.left -= PF;
+        kr.e -= PF;
+      } else {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int q = q0 - u;
+          if (q >= w0) {
+            const double xi = (double)cx[u].y - (double)cx[u].x * xn;
+            const int64_t sq = kr_prev(kr, S, M, cur, ev, nl, p);
+            if (q >= o0 && q <= o1) put_m(sq, (T)xi);
+            xn = xi;
+          }
         }
       }
 #pragma unroll
