@@ -640,8 +640,9 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
                          : ext_interval(S, ev, c.base + __popc(c.byte & ((2u << k) - 1u)), i0 + k,
                                         nl, nr, p, P) - c.e0;
       const int j1 = j + 1;
-      // FAST: j counts the knots at positions 1..k, so j <= k
-      const int lj = FAST ? k : 5, lj1 = FAST ? k + 1 : 5;
+      // FAST: j counts the knots at positions 1..k, i.e. the strict extrema
+      // of one envelope among k consecutive samples, so j <= (k + 1) / 2
+      const int lj = FAST ? (k + 1) / 2 : 5, lj1 = FAST ? (k + 1) / 2 + 1 : 5;
       const double v = spline_val(sel<NC + 1>(j, c.t0, c.t1, c.t2, c.t3, c.t4, c.t5, lj),
                                   sel<NC + 1>(j, c.y0, c.y1, c.y2, c.y3, c.y4, c.y5, lj),
                                   sel<NC + 1>(j, c.m0, c.m1, c.m2, c.m3, c.m4, c.m5, lj),
@@ -670,17 +671,19 @@ __device__ __forceinline__ void env_eval(const Shared<T, NW>& S, int ev, const E
         return __builtin_fmaf(tau, __builtin_fmaf(tau, __builtin_fmaf(tau, iv.c3, iv.c2), iv.c1),
                               iv.y);
       };
-      // j counts the knots at positions 1..k, so sample k has at most k + 1
-      // candidate intervals (k is a constant after unrolling)
+      // j counts the knots at positions 1..k, the strict extrema of one
+      // envelope among k consecutive samples (never two adjacent), so j <=
+      // (k + 1) / 2 (k is a constant after unrolling)
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
         const int j = __popc(c.byte & ((2u << k) - 2u));
         const float fk = (float)k;
         float v = horner(b0 + fk, v0);
-        if (k >= 1) v = (j == 1) ? horner(b1 + fk, v1) : v;
-        if constexpr (NC > 2) if (k >= 2) v = (j == 2) ? horner(b2 + fk, v2) : v;
-        if constexpr (NC > 3) if (k >= 3) v = (j == 3) ? horner(b3 + fk, v3) : v;
-        if constexpr (NC > 4) if (k >= 4) v = (j == 4) ? horner(b4 + fk, v4) : v;
+        const int jmax = (k + 1) / 2;
+        if (jmax >= 1) v = (j == 1) ? horner(b1 + fk, v1) : v;
+        if constexpr (NC > 2) if (jmax >= 2) v = (j == 2) ? horner(b2 + fk, v2) : v;
+        if constexpr (NC > 3) if (jmax >= 3) v = (j == 3) ? horner(b3 + fk, v3) : v;
+        if constexpr (NC > 4) if (jmax >= 4) v = (j == 4) ? horner(b4 + fk, v4) : v;
         acc[k] = ev == 0 ? v : acc[k] + v;
       }
       return;
