@@ -73,18 +73,21 @@ def test_fp64_three_tone_matches_oracle_and_reference(eng, golden):
     assert np.max(np.abs(out["resid"] - g["resid"])) <= 1e-10
 
 
-@pytest.mark.parametrize("B,S,threads", [(64, 30, 256), (100, 14, 1024), (1024, 30, 512),
-                                         (256, 30, 64)])
-def test_fp64_dsbt_matches_oracle(eng, B, S, threads):
+# B = 17 and 33: windows whose rows end inside a 16-row group, and (1024 threads) knot runs
+# that cross many segment regions inside a group, for the solve's straight-line row groups
+@pytest.mark.parametrize("B,S,threads,end", [(64, 30, 256, "mirror"), (100, 14, 1024, "mirror"),
+                                             (1024, 30, 512, "mirror"), (256, 30, 64, "mirror"),
+                                             (17, 5, 1024, "mirror"), (33, 30, 256, "natural")])
+def test_fp64_dsbt_matches_oracle(eng, B, S, threads, end):
     rng = np.random.default_rng(7)
     N = 20000
     t = np.arange(N)
     x = np.sin(2 * np.pi * t / 1500) + 0.4 * np.sin(2 * np.pi * t / 97) + 0.1 * rng.standard_normal(N)
     z = rng.standard_normal((3, N))
     sigma = 0.2 * np.std(x)
-    print("case: N=20000 E=3 fp64 DSBT B=%d S=%d threads=%d" % (B, S, threads), flush=True)
-    out = run1(eng, x, z, sigma, B=B, S=S, threads=threads, max_imf=6)
-    o = orc.eemd(x, z, sigma, tau=0.0, max_sifts=10, max_imf=6, end="mirror", B=B, S=S,
+    print("case: N=20000 E=3 fp64 DSBT B=%d S=%d threads=%d %s" % (B, S, threads, end), flush=True)
+    out = run1(eng, x, z, sigma, B=B, S=S, threads=threads, max_imf=6, end=end)
+    o = orc.eemd(x, z, sigma, tau=0.0, max_sifts=10, max_imf=6, end=end, B=B, S=S,
                  member_imfs=True, trace=True)
     assert np.array_equal(out["nmode"][0], o["nmode"])
     assert np.array_equal(out["trace"][0], o["trace"])
