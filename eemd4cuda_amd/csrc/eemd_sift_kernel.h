@@ -2217,7 +2217,7 @@ __global__ __launch_bounds__(NT, (sift_min_waves<T, NT>())) void sift_kernel(Sif
 // host-side launchers (called from the C-ABI)
 // ------------------------------------------------------------------------
 // Dynamic LDS of a launch: the BF16 boundary model in learned mode.
-static size_t sift_dyn_lds(int end_mode, int nt) {
+static inline size_t sift_dyn_lds(int end_mode, int nt) {
   return (end_mode == END_LEARNED && nt != WAVE) ? MLP_LDS_BYTES : 0;
 }
 
