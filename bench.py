@@ -295,8 +295,9 @@ def comparators(wl, fp, steps, halo, tune_block):
 
 
 def reference_sweep_leg():
-    """The reference's own GPU benchmark (its published wall times are in
-    BASELINE.md): EEMD of the 441,000-sample 10-octave sweep (refsignal.py,
+    """The reference's own GPU benchmark (published wall times:
+    Benchmark_Thomas_vs_Decay-Seamed_Block-Thomas.md:70-73, 150-153): EEMD of
+    the 441,000-sample 10-octave sweep (refsignal.py,
     emd_par.py:307-318) through decompose(method='gpu') with the reference's
     settings: numpy noise default_rng(0), eps 0.2, SD stop tau 0.25 (at most
     50 sifts), 16 IMFs, natural ends, block-Thomas B=250 / S=30, FP64.  Wall
