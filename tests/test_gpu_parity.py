@@ -19,7 +19,7 @@ def eng():
 
 
 def params(fp64=1, end="mirror", max_imf=8, max_sifts=10, tau=0.0, B=0, S=30, threads=0, mpt=1,
-           seed=0):
+           seed=0, min_extrema=4):
     from eemd4cuda_amd import _lib
     p = _lib.default_params()
     p.fp64 = fp64
@@ -32,6 +32,7 @@ def params(fp64=1, end="mirror", max_imf=8, max_sifts=10, tau=0.0, B=0, S=30, th
     p.threads = threads
     p.members_per_task = mpt
     p.seed = seed
+    p.min_extrema = min_extrema
     return p
 
 
@@ -93,6 +94,30 @@ def test_fp64_dsbt_matches_oracle(eng, B, S, threads, end):
     assert np.array_equal(out["trace"][0], o["trace"])
     assert np.max(np.abs(out["mimf"] - o["member_imfs"])) <= 1e-13
     assert np.max(np.abs(out["avg2"] - o["avg"])) <= 1e-13
+
+
+@pytest.mark.parametrize("min_extrema,end", [(12, "mirror"), (40, "natural")])
+def test_fp64_min_extrema_with_sd_stop_and_dsbt(eng, min_extrema, end):
+    """A residual with fewer than min_extrema extrema ends the member
+    (emd_ref.py:123-124), here with the SD stop (tau > 0) and DSBT windows: the
+    IMF count, every sift's extrema and the IMFs equal the oracle's."""
+    rng = np.random.default_rng(11)
+    N = 6000
+    t = np.arange(N)
+    x = np.sin(2 * np.pi * t / 900) + 0.5 * np.sin(2 * np.pi * t / 61) + 0.05 * rng.standard_normal(N)
+    z = rng.standard_normal((4, N))
+    sigma = 0.2 * np.std(x)
+    print("case: N=6000 E=4 fp64 tau=0.2 B=128 min_extrema=%d %s" % (min_extrema, end), flush=True)
+    out = run1(eng, x, z, sigma, tau=0.2, max_sifts=30, max_imf=12, B=128, S=30, end=end,
+               min_extrema=min_extrema)
+    o = orc.eemd(x, z, sigma, tau=0.2, max_sifts=30, max_imf=12, min_extrema=min_extrema,
+                 end=end, B=128, S=30, member_imfs=True, trace=True)
+    assert np.array_equal(out["nmode"][0], o["nmode"])
+    assert np.array_equal(out["nsift"][0], o["nsift"])
+    assert np.array_equal(out["trace"][0], o["trace"])
+    assert np.max(np.abs(out["mimf"] - o["member_imfs"])) <= 1e-13
+    # the stop rule bites: fewer IMFs than with the default threshold
+    assert int(out["n_modes"][0]) < 12
 
 
 def test_fp64_natural_end_matches_reference(eng, golden):
