@@ -20,8 +20,8 @@ CSRC = PKG / "csrc"
 OUT_DIR = PKG / "_build"
 OBJ_DIR = OUT_DIR / "obj"
 LIB_NAME = "libeemd_mi355x.so"
-SOURCES = ["eemd_sift_f64.hip", "eemd_sift_f32.hip", "eemd_sift.hip", "eemd_mlp.hip",
-           "eemd_spectra.hip", "eemd_capi.cpp"]
+SOURCES = ["eemd_sift_f64.hip", "eemd_sift_f32.hip", "eemd_sift_f32w.hip", "eemd_sift.hip",
+           "eemd_mlp.hip", "eemd_spectra.hip", "eemd_capi.cpp"]
 
 # -ffp-contract=off: the FP64 path reproduces the oracle's rounding (explicit
 # fma() only where intended).  Packed FP32 instructions are disabled.
@@ -31,9 +31,10 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-# Per-source flags.  The FP32 sift kernels (two waves per SIMD) run about 3%
-# faster with the ILP-maximising instruction scheduler; the FP64 kernels (one
-# wave per SIMD, register-bound) are slower with it and keep the default.
+# Per-source flags.  The FP32 sift kernels with a workgroup per member run
+# about 3% faster with the ILP-maximising instruction scheduler; the FP64
+# kernels (one wave per SIMD, register-bound) and the FP32 one-wave kernel
+# (its learned-ends path) are slower with it and keep the default.
 SOURCE_FLAGS = {
     "eemd_sift_f32.hip": ["-mllvm", "-amdgpu-sched-strategy=max-ilp"],
 }

@@ -2240,45 +2240,16 @@ static hipError_t launch_sift_t(const SiftArgs& A, int grid, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Launch / occupancy of one precision's sift kernel by workgroup size.
-template <typename T>
-static hipError_t launch_sift_prec(const SiftArgs& A, int nt, int grid, hipStream_t st) {
-  switch (nt) {
-    case 64: return launch_sift_t<T, 64>(A, grid, st);
-    case 256: return launch_sift_t<T, 256>(A, grid, st);
-    case 512: return launch_sift_t<T, 512>(A, grid, st);
-    case 1024: return launch_sift_t<T, 1024>(A, grid, st);
+// Occupancy (workgroups per CU) of one sift kernel instantiation.
+template <typename T, int NT>
+static hipError_t sift_occupancy_t(int end_mode, int* blocks_per_cu) {
+  const size_t lds = sift_dyn_lds(end_mode, NT);
+  if (lds) {
+    const hipError_t e = allow_model_lds<T, NT>();
+    if (e != hipSuccess) return e;
   }
-  return hipErrorInvalidValue;
-}
-
-template <typename T>
-static hipError_t sift_occupancy_prec(int nt, int end_mode, int* blocks_per_cu) {
-  const void* f = nullptr;
-  hipError_t e = hipSuccess;
-  const bool lds = sift_dyn_lds(end_mode, nt) != 0;
-  switch (nt) {
-    case 64:
-      f = (const void*)sift_kernel<T, 64>;
-      if (lds) e = allow_model_lds<T, 64>();
-      break;
-    case 256:
-      f = (const void*)sift_kernel<T, 256>;
-      if (lds) e = allow_model_lds<T, 256>();
-      break;
-    case 512:
-      f = (const void*)sift_kernel<T, 512>;
-      if (lds) e = allow_model_lds<T, 512>();
-      break;
-    case 1024:
-      f = (const void*)sift_kernel<T, 1024>;
-      if (lds) e = allow_model_lds<T, 1024>();
-      break;
-  }
-  if (!f) return hipErrorInvalidValue;
-  if (e != hipSuccess) return e;
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, nt,
-                                                      sift_dyn_lds(end_mode, nt));
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      blocks_per_cu, reinterpret_cast<const void*>(sift_kernel<T, NT>), NT, lds);
 }
 
 }  // namespace eemd
