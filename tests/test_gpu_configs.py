@@ -14,6 +14,8 @@
   (packaged model) in FP64 equal the oracle with the same model.
 * the reference's first-IMF EEMD gpu_eemd (emd_gpu.py:337-358) equals the
   oracle's first-IMF member mean.
+* 300 short signals of ragged lengths with natural ends in one call (one wave
+  per member): picked members equal the oracle bit for bit.
 """
 import numpy as np
 import pytest
@@ -229,3 +231,44 @@ def test_gpu_eemd_first_imf_matches_oracle():
     assert np.array_equal(nsift, o["nsift"])
     assert np.max(np.abs(raw - o["member_imfs"][:, 0])) <= 1e-12
     assert np.max(np.abs(avg - o["member_imfs"][:, 0].mean(axis=0))) <= 1e-12
+
+
+def test_short_ragged_batch_natural_ends_matches_oracle(eng):
+    """Hundreds of short signals of ragged lengths (40 .. 3000 samples, most
+    shorter than a wave's 512-sample chunk) in one call with natural ends: the
+    launch takes one wave per member, and a picked member's extrema of every
+    sift, IMF and sift counts and IMFs equal the oracle's (FP64, Philox noise,
+    exact Thomas)."""
+    rng = np.random.default_rng(21)
+    lens = rng.integers(40, 3000, size=300)
+    xs = []
+    for n in lens:
+        t = np.arange(n)
+        f1, f2 = rng.uniform(0.005, 0.2, size=2)
+        xs.append(np.sin(2 * np.pi * f1 * t) + 0.4 * np.sin(2 * np.pi * f2 * t + 1.0)
+                  + 0.05 * rng.standard_normal(n))
+    offs = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    sig = np.array([0.2 * np.std(v) for v in xs])
+    E, mi = 8, 6
+    print("case: 300 short ragged signals E=8 fp64 natural, one call", flush=True)
+    out = eng.run_batch(np.concatenate(xs), offs, E, sig,
+                        params(max_imf=mi, end="natural", B=0, mpt=0, seed=7),
+                        member_imfs=True, trace=True)
+    assert eng.last_timing()["threads"] == 64
+    ms = means(out, lens, mi)
+    for s in range(len(xs)):
+        r = out["resid"][offs[s]:offs[s + 1]]
+        assert np.max(np.abs(ms[s].sum(axis=0) + r - xs[s])) <= 1e-12 * np.max(np.abs(xs[s])), s
+    order = np.argsort(lens)
+    pick = order[np.linspace(0, len(xs) - 1, 10).astype(int)]
+    mimf = out["member_imfs"]
+    for s in pick:
+        n = int(lens[s])
+        z = np.stack([orc.normals(7, int(s), e, n) for e in range(E)])
+        o = orc.eemd(xs[s], z, sig[s], tau=0.0, max_sifts=10, max_imf=mi, end="natural", B=0,
+                     member_imfs=True, trace=True)
+        assert np.array_equal(out["nmode"][s], o["nmode"]), s
+        assert np.array_equal(out["nsift"][s], o["nsift"]), s
+        assert np.array_equal(out["trace"][s], o["trace"]), s
+        got = mimf[E * mi * offs[s]: E * mi * offs[s + 1]].reshape(E, mi, n)
+        assert np.max(np.abs(got - o["member_imfs"])) <= 1e-13 * max(1.0, np.max(np.abs(xs[s]))), s
