@@ -304,18 +304,22 @@ int make_plan(eemd_ctx* c, const int64_t* offsets, int32_t nsig, int32_t E, cons
   const int64_t max_slots = (int64_t)bpc * c->ncu;
 
   // Members per task.  An explicit value applies to every signal.  The
-  // automatic rule is per signal: a task of signal s holds at most
-  // W / (2 * slots * n_s) members (W = total sample-members), so no task is
-  // more than half of one slot's fair share and the longest-first queue can
-  // balance the tail, while short signals keep few tasks and few
-  // accumulators (one max_imf x n_s FP64 block per task).
+  // automatic value is one count for the whole batch: the most members whose
+  // task on the longest signal is at most one slot's fair share of the work
+  // (W / slots sample-members, W = total sample-members).  Tasks of shorter
+  // signals are proportionally smaller and come last in the longest-first
+  // queue, where they fill the tail.  The accumulators (one max_imf x n_s
+  // FP64 block per task) come to about slots x max_imf x n_max doubles.
+  // A short member costs more than its sample count says (each sift's window
+  // solve is a serial chain of up to B + 2S rows however short the signal),
+  // so a rule that gave short signals few large tasks left those tasks to
+  // run last and the launch idle for a quarter of its time on batches of
+  // uneven lengths.
   int64_t work = 0;
   for (int s = 0; s < nsig; ++s) work += (offsets[s + 1] - offsets[s]) * (int64_t)E;
-  auto mpt_of = [&](int64_t n) -> int {
-    if (want_mpt > 0) return std::min(want_mpt, E);
-    const int64_t m = work / (2 * max_slots * std::max<int64_t>(n, 1));
-    return (int)std::max<int64_t>(1, std::min<int64_t>(m, E));
-  };
+  const int mpt_auto = (int)std::max<int64_t>(
+      1, std::min<int64_t>(work / (max_slots * std::max<int64_t>(P.nmax, 1)), E));
+  auto mpt_of = [&](int64_t) -> int { return want_mpt > 0 ? std::min(want_mpt, E) : mpt_auto; };
   P.mpt = want_mpt > 0 ? std::min(want_mpt, E) : 0;
 
   P.sig.resize(nsig);

@@ -122,7 +122,7 @@ def test_autotune_choice_is_applied_and_exact(eng, fp64):
     base = _dev_run(eng, xd, offs, E, sig, p, mi)
     best, ms = eng.autotune(xd.data_ptr(), offs, E, sig, p, reps=1)
     assert best.threads in (64, 256, 512, 1024)
-    assert 0 <= best.members_per_task <= E  # 0: the automatic per-signal rule
+    assert 0 <= best.members_per_task <= E  # 0: the automatic rule
     assert ms > 0
     if not fp64:
         assert best.wave_windows != 0
