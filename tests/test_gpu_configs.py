@@ -16,6 +16,8 @@
   oracle's first-IMF member mean.
 * 300 short signals of ragged lengths with natural ends in one call (one wave
   per member): picked members equal the oracle bit for bit.
+* the automatic members-per-task count is one for the whole batch and only
+  regroups the ensemble sum.
 """
 import numpy as np
 import pytest
@@ -272,3 +274,41 @@ def test_short_ragged_batch_natural_ends_matches_oracle(eng):
         assert np.array_equal(out["trace"][s], o["trace"]), s
         got = mimf[E * mi * offs[s]: E * mi * offs[s + 1]].reshape(E, mi, n)
         assert np.max(np.abs(got - o["member_imfs"])) <= 1e-13 * max(1.0, np.max(np.abs(xs[s]))), s
+
+
+def test_automatic_task_count_is_one_for_the_batch(eng):
+    """Automatic members per task on a batch of uneven lengths: one count for
+    every signal (so the queue holds the same number of tasks per signal),
+    taken from the longest signal.  Tasks only regroup the ensemble sum: the
+    members' IMFs and sift counts equal those of one member per task bit for
+    bit, and the ensemble means equal them to re-association."""
+    rng = np.random.default_rng(33)
+    lens = rng.integers(100, 2000, size=200)
+    xs = []
+    for n in lens:
+        t = np.arange(n)
+        f1, f2 = rng.uniform(0.01, 0.2, size=2)
+        xs.append(np.sin(2 * np.pi * f1 * t) + 0.5 * np.sin(2 * np.pi * f2 * t)
+                  + 0.05 * rng.standard_normal(n))
+    offs = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    sig = np.array([0.2 * np.std(v) for v in xs])
+    x = np.concatenate(xs)
+    E, mi = 32, 4
+    print("case: 200 uneven signals E=32 fp64, automatic vs one member per task", flush=True)
+    auto = eng.run_batch(x, offs, E, sig, params(max_imf=mi, max_sifts=5, mpt=0, seed=3),
+                         member_imfs=True)
+    t_auto = eng.last_timing()
+    one = eng.run_batch(x, offs, E, sig, params(max_imf=mi, max_sifts=5, mpt=1, seed=3),
+                        member_imfs=True)
+    nsig = len(lens)
+    assert t_auto["ntasks"] % nsig == 0
+    per_signal = t_auto["ntasks"] // nsig
+    assert 1 < per_signal < E  # several members per task, the same for every length
+    assert np.array_equal(auto["member_imfs"], one["member_imfs"])
+    assert np.array_equal(auto["nmode"], one["nmode"])
+    assert np.array_equal(auto["nsift"], one["nsift"])
+    assert np.array_equal(auto["n_modes"], one["n_modes"])
+    ma, m1 = means(auto, lens, mi), means(one, lens, mi)
+    for s in range(nsig):
+        scale = np.max(np.abs(xs[s]))
+        assert np.max(np.abs(ma[s] - m1[s])) <= 1e-13 * scale, s
