@@ -18,7 +18,7 @@ reference's CPU path, restated in numpy (oracle/emd_np.py), on a bounded
 sample of members of the same workload, one member per host core.
 
     python bench.py [--steps 5] [--warmup 1] [--workload audio_mix|three_tone|uneven|chirp_scale]
-                    [--only f64|f32] [--no-legs] [--no-cpu]
+                    [--only f64|f32 [--end mirror|learned]] [--no-legs] [--no-cpu]
 
 Besides the headline, the default run reports two bounded legs of the
 metric's own comparison: plain per-signal launches against the batched,
@@ -158,9 +158,11 @@ class DeviceRun:
         return out
 
 
-def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0):
+def time_mode(wl, fp, steps, warmup, halo, threads=0, mpt=0, wave_windows=0, end="mirror"):
     import torch
+    from eemd4cuda_amd import _lib
     r = DeviceRun(wl, fp, halo, threads=threads, mpt=mpt, wave_windows=wave_windows)
+    r.p.end_mode = _lib.END_MODES[end]
     for _ in range(warmup):
         r.step()
     torch.cuda.synchronize()
@@ -431,6 +433,8 @@ def main():
     ap.add_argument("--E", type=int, default=0, help="override the ensemble size")
     ap.add_argument("--max-imf", type=int, default=0, help="override the IMF count")
     ap.add_argument("--wave-windows", type=int, default=0, help="wave-parallel solve threshold")
+    ap.add_argument("--end", choices=("mirror", "learned"), default="mirror",
+                    help="end extension of an --only run (learned: the packaged model)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters)")
@@ -453,9 +457,13 @@ def main():
         wl["max_imf"] = a.max_imf
     halo = {"f64": 30, "f32": a.halo32}
     if a.only:
+        if a.end == "learned":
+            from eemd4cuda_amd import learned
+            learned.install(learned.load(learned.default_model_path()))
         res, run = time_mode(wl, a.only, a.steps, a.warmup, halo=halo[a.only], threads=a.threads,
-                             mpt=a.mpt, wave_windows=a.wave_windows)
-        line = dict(fp=a.only, workload=wl["label"],
+                             mpt=a.mpt, wave_windows=a.wave_windows, end=a.end)
+        label = wl["label"] if a.end == "mirror" else wl["label"].replace("mirror", "learned ends")
+        line = dict(fp=a.only, workload=label, end=a.end,
                     **{k: res[k] for k in ("value", "ms_per_step", "sift_kernel_ms",
                                            "threads", "grid", "roofline")})
         if a.phases:
