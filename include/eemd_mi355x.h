@@ -76,7 +76,9 @@ typedef struct eemd_params {
   int32_t block;            /* decay-seamed block-Thomas block B; <= 0: exact Thomas */
   int32_t halo;             /* seam halo S in knots */
   int32_t threads;          /* workgroup size 64 / 256 / 512 / 1024; 0: automatic */
-  int32_t members_per_task; /* ensemble members per queue entry; 0: automatic */
+  int32_t members_per_task; /* ensemble members per queue entry; 0: automatic (one count for
+                               the batch: the most members whose task on the longest signal
+                               is at most one resident workgroup's share of the work) */
   int32_t flags;            /* EEMD_FLAG_* */
   double tau;               /* SD stopping tolerance (emd_ref.py:140-146); <= 0: fixed sifts */
   uint64_t seed;            /* Philox key when no explicit noise is given */
