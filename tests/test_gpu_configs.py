@@ -18,6 +18,8 @@
   per member): picked members equal the oracle bit for bit.
 * the automatic members-per-task count is one for the whole batch and only
   regroups the ensemble sum.
+* the SD stop (tau > 0) in a one-wave-per-member batch with DSBT windows:
+  picked members equal the oracle bit for bit.
 """
 import numpy as np
 import pytest
@@ -312,3 +314,43 @@ def test_automatic_task_count_is_one_for_the_batch(eng):
     for s in range(nsig):
         scale = np.max(np.abs(xs[s]))
         assert np.max(np.abs(ma[s] - m1[s])) <= 1e-13 * scale, s
+
+
+def test_one_wave_batch_with_sd_stop_matches_oracle(eng):
+    """The SD stop (tau > 0: each IMF ends at the first sift whose SD is below
+    tau, emd_ref.py:140-146) in a batched launch of one wave per member, with
+    DSBT windows and mirror ends: picked members' extrema of every sift, sift
+    and IMF counts and IMFs equal the oracle's (FP64, Philox noise)."""
+    rng = np.random.default_rng(41)
+    lens = rng.integers(500, 6000, size=150)
+    xs = []
+    for n in lens:
+        t = np.arange(n)
+        f1, f2, f3 = rng.uniform(0.002, 0.15, size=3)
+        xs.append(np.sin(2 * np.pi * f1 * t) + 0.6 * np.sin(2 * np.pi * f2 * t + 0.3)
+                  + 0.3 * np.sin(2 * np.pi * f3 * t) + 0.05 * rng.standard_normal(n))
+    offs = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    sig = np.array([0.2 * np.std(v) for v in xs])
+    E, mi, tau, ms = 16, 8, 0.2, 30
+    p = params(max_imf=mi, max_sifts=ms, B=256, S=30, mpt=0, seed=9)
+    p.tau = tau
+    print("case: 150 signals E=16 fp64 tau=0.2 B=256 mirror, one wave per member", flush=True)
+    out = eng.run_batch(np.concatenate(xs), offs, E, sig, p, member_imfs=True, trace=True)
+    assert eng.last_timing()["threads"] == 64
+    mimf = out["member_imfs"]
+    order = np.argsort(lens)
+    pick = order[np.linspace(0, len(xs) - 1, 6).astype(int)]
+    for s in pick:
+        n = int(lens[s])
+        z = np.stack([orc.normals(9, int(s), e, n) for e in range(E)])
+        o = orc.eemd(xs[s], z, sig[s], tau=tau, max_sifts=ms, max_imf=mi, end="mirror", B=256,
+                     S=30, member_imfs=True, trace=True)
+        assert np.array_equal(out["nmode"][s], o["nmode"]), s
+        assert np.array_equal(out["nsift"][s], o["nsift"]), s
+        assert np.array_equal(out["trace"][s], o["trace"]), s
+        got = mimf[E * mi * offs[s]: E * mi * offs[s + 1]].reshape(E, mi, n)
+        for e in range(E):
+            for m in range(int(o["nmode"][e])):
+                assert rel_l2(got[e, m], o["member_imfs"][e, m]) <= 1e-10, (s, e, m)
+    # sift counts vary with the SD stop
+    assert len(np.unique(out["nsift"])) > 1
