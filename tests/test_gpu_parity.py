@@ -160,6 +160,44 @@ def test_degenerate_inputs_match_reference(eng, golden):
         assert np.allclose(out["mimf"][0, :nm], g[k + "_imfs"], rtol=0, atol=1e-12), k
 
 
+@pytest.mark.parametrize("end", ["natural", "mirror"])
+def test_degenerate_inputs_in_a_one_wave_batch(eng, golden, end):
+    """The same degenerate inputs (constant, monotone, plateaus, minima only,
+    two knots, 8 samples, alternating) and the shortest legal signals (3 to 7
+    samples) in one batched call with enough members (zero noise) to take one
+    wave per member: every member equals the reference's golden EMD, and the
+    short signals equal the oracle."""
+    g = golden("golden_degenerate.npz")
+    names = sorted(k[:-2] for k in g.files if k.endswith("_x") and k[:-2].endswith(end))
+    rng = np.random.default_rng(5)
+    shorts = [rng.standard_normal(n) for n in (3, 4, 5, 6, 7)]
+    xs = [g[k + "_x"] for k in names] + shorts
+    lens = np.array([len(v) for v in xs], dtype=np.int64)
+    offs = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    E, mi = 180, 6
+    print("case: degenerate + short signals in one batch, E=180, %s" % end, flush=True)
+    out = eng.run_batch(np.concatenate(xs), offs, E, np.zeros(len(xs)),
+                        params(end=end, max_imf=mi, B=0, mpt=0), member_imfs=True)
+    assert eng.last_timing()["threads"] == 64
+    mimf = out["member_imfs"]
+    for s, v in enumerate(xs):
+        n = len(v)
+        got = mimf[E * mi * offs[s]: E * mi * offs[s + 1]].reshape(E, mi, n)
+        if s < len(names):
+            k = names[s]
+            ref, pm = g[k + "_imfs"], g[k + "_per_mode"]
+        else:
+            o = orc.eemd(v, np.zeros((1, n)), 0.0, tau=0.0, max_sifts=10, max_imf=mi, end=end,
+                         member_imfs=True)
+            k = "short%d" % n
+            ref, pm = o["member_imfs"][0, :o["nmode"][0]], o["per_mode"][0, :o["nmode"][0]]
+        nm = ref.shape[0]
+        assert np.all(out["nmode"][s] == nm), k
+        assert np.all(out["per_mode"][s][:, :nm] == pm), k
+        for e in (0, E // 2, E - 1):
+            assert np.allclose(got[e, :nm], ref, rtol=0, atol=1e-12), (k, e)
+
+
 def test_philox_noise_matches_oracle(eng):
     print("case: philox normals", flush=True)
     z = eng.noise(2024, 5, 3, 4, 1001)
