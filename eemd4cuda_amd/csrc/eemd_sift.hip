@@ -5,16 +5,15 @@
 
 namespace eemd {
 
-// Level-wise ensemble mean and residual (emd_ref.py:209-217): thread per
-// sample; task accumulators of a signal are added in task (= member) order.
+// Level-wise ensemble mean and residual (emd_ref.py:209-217) of one sample:
+// the task accumulators of its signal are added in task (= member) order.
 // With `sums` the levels hold the sums (not divided by the member counts), for
 // callers that combine several batches of one ensemble.
-__global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_start,
-                            const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
-                            int max_imf, const double* x, double* avg, double* resid,
-                            int32_t* nmodes, int64_t total_n, int sums) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= total_n) return;
+__device__ __forceinline__ void mean_sample(const SignalDesc* sig, int nsig, const int64_t* sig_start,
+                                            const TaskDesc* task, const double* acc,
+                                            const int32_t* acc_cnt, int max_imf, const double* x,
+                                            double* avg, double* resid, int32_t* nmodes,
+                                            int64_t gid, int sums) {
   int lo = 0, hi = nsig - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -46,6 +45,85 @@ __global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_
   }
   resid[sd.x_off + i] = x[sd.x_off + i] - (nm > 0 ? rs : 0.0);
   if (i == 0) nmodes[lo] = nm;
+}
+
+// One wave per MEAN_SPW consecutive samples, lane l taking samples l, l + 64,
+// ... of them.  When they all belong to one signal (every wave of a long
+// signal), the signal and the tasks' counts and accumulator offsets are the
+// same for the whole wave and are read once into scalar registers, so each
+// accumulator value costs one coalesced vector load; a wave that straddles
+// signals runs the per-sample form.  The sums and their order are those of
+// mean_sample.
+constexpr int MEAN_SPL = 4;                   // samples per lane
+constexpr int MEAN_SPW = MEAN_SPL * WAVE;     // samples per wave
+
+__global__ void mean_kernel(const SignalDesc* sig, int nsig, const int64_t* sig_start,
+                            const TaskDesc* task, const double* acc, const int32_t* acc_cnt,
+                            int max_imf, const double* x, double* avg, double* resid,
+                            int32_t* nmodes, int64_t total_n, int sums) {
+  const int ln = (int)(threadIdx.x % WAVE);
+  const int64_t base = ((int64_t)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE) * MEAN_SPW;
+  if (base >= total_n) return;  // wave-uniform
+  const int64_t wend = (base + MEAN_SPW < total_n) ? base + MEAN_SPW : total_n;
+  int lo = 0, hi = nsig - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sig_start[mid] <= base) lo = mid; else hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const SignalDesc sd = sig[lo];
+  const int64_t s0 = sig_start[lo];
+  if (wend > s0 + sd.n) {
+#pragma unroll 1
+    for (int k = 0; k < MEAN_SPL; ++k) {
+      const int64_t gid = base + ln + (int64_t)k * WAVE;
+      if (gid < total_n)
+        mean_sample(sig, nsig, sig_start, task, acc, acc_cnt, max_imf, x, avg, resid, nmodes,
+                    gid, sums);
+    }
+    return;
+  }
+  const int64_t i0 = base - s0 + ln;  // sample of k = 0 within the signal
+  bool ok[MEAN_SPL];
+#pragma unroll
+  for (int k = 0; k < MEAN_SPL; ++k) ok[k] = base + ln + (int64_t)k * WAVE < wend;
+  double rs[MEAN_SPL];
+#pragma unroll
+  for (int k = 0; k < MEAN_SPL; ++k) rs[k] = 0.0;
+  int nm = 0;
+  for (int m = 0; m < max_imf; ++m) {
+    int cnt = 0;
+    double s[MEAN_SPL];
+#pragma unroll
+    for (int k = 0; k < MEAN_SPL; ++k) s[k] = 0.0;
+    const int64_t mo = (int64_t)m * sd.n + i0;
+#pragma unroll 4
+    for (int t = sd.task0; t < sd.task0 + sd.ntask; ++t) {
+      const int c = acc_cnt[(int64_t)t * max_imf + m];
+      const double* const a = acc + task[t].acc_off + mo;
+#pragma unroll
+      for (int k = 0; k < MEAN_SPL; ++k) {
+        const double v = ok[k] ? a[k * WAVE] : 0.0;
+        s[k] = (c > 0) ? s[k] + v : s[k];
+      }
+      cnt += c;
+    }
+    if (cnt == 0) break;  // wave-uniform
+#pragma unroll
+    for (int k = 0; k < MEAN_SPL; ++k) {
+      const double av = sums ? s[k] : s[k] / (double)cnt;
+      if (ok[k]) avg[sd.avg_off + mo + k * WAVE] = av;
+      rs[k] = (m == 0) ? av : rs[k] + av;
+    }
+    nm = m + 1;
+  }
+#pragma unroll
+  for (int k = 0; k < MEAN_SPL; ++k) {
+    if (!ok[k]) continue;
+    const int64_t q = sd.x_off + i0 + k * WAVE;
+    resid[q] = x[q] - (nm > 0 ? rs[k] : 0.0);
+  }
+  if (base == s0 && ln == 0) nmodes[lo] = nm;
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
@@ -98,7 +176,8 @@ hipError_t launch_mean(const SignalDesc* sig, int nsig, const int64_t* sig_start
                        int32_t* nmodes, int64_t total_n, int sums, hipStream_t st) {
   if (total_n <= 0) return hipSuccess;
   const int bs = 256;
-  const int64_t g = (total_n + bs - 1) / bs;
+  const int64_t per_block = (int64_t)(bs / WAVE) * MEAN_SPW;
+  const int64_t g = (total_n + per_block - 1) / per_block;
   hipLaunchKernelGGL(mean_kernel, dim3((unsigned)g), dim3(bs), 0, st, sig, nsig, sig_start,
                      task, acc, acc_cnt, max_imf, x, avg, resid, nmodes, total_n, sums);
   return hipGetLastError();
