@@ -14,8 +14,8 @@
  *   emd_ref.py:67-75   _spline_eval              -> orc_spline_eval()
  *   emd_ref.py:77-103  envelope (natural/mirror) -> orc_envelope()
  *   emd_ref.py:108-156 emd_1d                    -> orc_emd_1d()
- *   emd_ref.py:199-217 eemd (level-wise mean, max_imf honoured per member as
- *                      the survey's oracle definition requires)      -> orc_eemd()
+ *   emd_ref.py:199-217 eemd (level-wise mean; max_imf is honoured per member,
+ *                      as emd_par.py:148, 237 pass it to each one)   -> orc_eemd()
  *   emd_ref.py:141-142 np.sum (8192-element blocks, pairwise inside)  -> orc_pairwise_sum()
  * plus the counter-based noise the GPU path generates in-kernel:
  *   Philox4x32-10 (Salmon et al., SC'11) + Box-Muller                -> orc_philox4x32_10(), orc_normals()
